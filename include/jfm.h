@@ -1,0 +1,184 @@
+/*
+ * jfm.h -- C ABI of the MI355X-native FMM evaluation path (libjfm.so).
+ *
+ * Framework-neutral: plain pointers, sizes and an opaque stream handle
+ * (a hipStream_t passed as void*; NULL is the legacy default stream).  No
+ * exceptions cross this boundary: every function returns JFM_OK (0) or an
+ * error code, and jfm_last_error() returns a thread-local message.
+ *
+ * Device entry points are stream-ordered, never synchronise the host and never
+ * allocate: every buffer (including scratch) is owned by the caller.  Outputs
+ * are written in place and must not alias inputs.
+ *
+ * dtype selects the working precision of a device entry point:
+ *   JFM_F32 -- float32 buffers; JFM_F64 -- float64 buffers.
+ *
+ * Layouts (all row-major, tree order = Morton order of the topology):
+ *   pos4      [N][4]       x, y, z, mass per particle in tree order
+ *   centers4  [nodes][4]   centre of mass x, y, z and total mass per node
+ *   coeffs    [nodes][cs]  packed real (Dehnen no-sqrt2) coefficients,
+ *                          index n*n + n + m, row stride cs = jfm_coeff_stride()
+ *   node ids: internal nodes [0, L-1) with the root at 0, leaf l is node L-1+l;
+ *             leaf l owns tree-order particles [l*leaf_size, min(N,(l+1)*leaf_size)).
+ *
+ * Each entry point names the reference interface it replaces (TobiBu/jaccpot).
+ */
+#ifndef JFM_H
+#define JFM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define JFM_OK 0
+#define JFM_ERR_ARG 1          /* invalid argument / unsupported size */
+#define JFM_ERR_HIP 2          /* HIP runtime error (launch or API) */
+#define JFM_ERR_UNSUPPORTED 3  /* order / dtype combination not compiled */
+#define JFM_ERR_ALLOC 4        /* host allocation failure (topology build) */
+
+#define JFM_F32 0
+#define JFM_F64 1
+
+/* evaluate flags */
+#define JFM_EVAL_NEAR 1         /* add the P2P near field (self leaf + near list) */
+#define JFM_EVAL_FAR 2          /* add the L2P far field from the leaf locals */
+#define JFM_EVAL_POTENTIAL 4    /* also write potentials */
+#define JFM_EVAL_COMPENSATED 8  /* blocked Kahan accumulation of the P2P sums */
+
+#define JFM_MAX_ORDER 8
+
+/* Thread-local message for the last failing call on this thread. */
+const char* jfm_last_error(void);
+/* ABI version (bumped on any signature change). */
+int32_t jfm_abi_version(void);
+/* Row stride (elements) of a coefficient array for order p and dtype. */
+int32_t jfm_coeff_stride(int32_t p, int32_t dtype);
+/* 1 if kernels for (p, dtype) are compiled into this library. */
+int32_t jfm_order_supported(int32_t p, int32_t dtype);
+
+/* ------------------------------------------------------------------------
+ * Host topology (replaces yggdrax build_tree + compute_tree_mass_moments +
+ * compute_tree_geometry + build_interactions_and_neighbors as called from
+ * jaccpot/runtime/fmm_state.py:630-800, upward/real_tree_expansions.py:439-440,
+ * upward/tree_geometry.py:65-127 and runtime/_interaction_cache.py:523,564).
+ * ---------------------------------------------------------------------- */
+typedef struct jfm_topology jfm_topology;
+
+/* Build the fixed static-radix topology and its far/near lists on the host.
+ * positions [n][3] and masses [n] in input order (float64).  bounds is
+ * {lo_x, lo_y, lo_z, hi_x, hi_y, hi_z} or NULL for the bounding box.
+ * num_threads <= 0 uses all hardware threads. */
+int32_t jfm_topology_build(const double* positions, const double* masses, int64_t n, int32_t leaf_size,
+                           double theta, const double* bounds, int32_t num_threads, jfm_topology** out);
+
+/* info[0..8] = N, leaf_size, num_leaves, num_internal, num_nodes, far pairs K,
+ *              near entries E, max depth, number of targets with far sources. */
+int32_t jfm_topology_info(const jfm_topology* topo, int64_t* info);
+
+/* Copy the topology arrays into caller buffers (any pointer may be NULL).
+ *   perm[N]        tree slot -> input index
+ *   parent[nodes]  (-1 at the root), left[I], right[I]
+ *   node_range[nodes][2] inclusive tree-order particle range, depth[nodes]
+ *   center[nodes][3], radius[nodes], mass[nodes]  (float64 geometry of the build)
+ *   far_off[nodes+1], far_src[K]  far list by target node, sources ascending
+ *   near_off[L+1], near_src[E]    near list by target leaf (leaf indices, self excluded) */
+int32_t jfm_topology_export(const jfm_topology* topo, int32_t* perm, int32_t* parent, int32_t* left,
+                            int32_t* right, int32_t* node_range, int32_t* depth, double* center,
+                            double* radius, double* mass, int32_t* far_off, int32_t* far_src,
+                            int32_t* near_off, int32_t* near_src);
+
+/* Level schedules: m2m_nodes = internal nodes by depth, deepest level first, with
+ * m2m_off[levels+1]; l2l_nodes = non-root nodes by depth, shallowest first, with
+ * l2l_off[levels+1]; m2l_targets = nodes that have far sources (ascending).
+ * Sizes: I, max_depth+1, nodes-1, max_depth+1, info[8]. */
+int32_t jfm_topology_schedules(const jfm_topology* topo, int32_t* m2m_nodes, int32_t* m2m_off,
+                               int32_t* l2l_nodes, int32_t* l2l_off, int32_t* m2l_targets);
+
+void jfm_topology_free(jfm_topology* topo);
+
+/* Dehnen real multipole swap matrices B_U for degrees 0..p, float64, packed
+ * row-major block after block (sum (2l+1)^2 entries); the constants every
+ * rotation kernel is compiled with (jaccpot/operators/real_dehnen_q.py:343-470). */
+int32_t jfm_tables_bu(int32_t p, double* out, int64_t capacity);
+
+/* ------------------------------------------------------------------------
+ * Device kernels.
+ * ---------------------------------------------------------------------- */
+
+/* Gather input-order positions/masses into tree-order pos4.
+ * src_dtype describes positions [n][3] / masses [n]; dtype describes pos4. */
+int32_t jfm_gather_sorted(void* stream, int32_t src_dtype, int32_t dtype, const void* positions,
+                          const void* masses, const int32_t* perm, int32_t n, void* pos4);
+
+/* Leaf P2M about each leaf's centre of mass; also writes the leaves' centers4.
+ * Replaces _p2m_leaves_real (upward/real_tree_expansions.py:89-213) with the
+ * centres of yggdrax compute_tree_mass_moments (:439-440). */
+int32_t jfm_p2m(void* stream, int32_t dtype, int32_t p, const void* pos4, int32_t n, int32_t num_leaves,
+                int32_t leaf_size, int32_t num_internal, void* multipoles, void* centers4);
+
+/* One M2M level: for each internal node in nodes[count], combine its children
+ * (centre of mass, multipole = m2m(left) + m2m(right)); node_range [nodes][2]
+ * (inclusive tree-order particle ranges) weights the mean of a massless node.
+ * Replaces aggregate_m2m_real_by_level (upward/real_tree_expansions.py:216-364). */
+int32_t jfm_m2m_level(void* stream, int32_t dtype, int32_t p, const int32_t* nodes, int32_t count,
+                      const int32_t* left, const int32_t* right, const int32_t* node_range, void* multipoles,
+                      void* centers4);
+
+/* M2L over the far list: locals[t] = sum_{s in far(t)} m2l(M[s], c_t - c_s) for
+ * every t in targets[num_targets] (written, not accumulated; other rows untouched).
+ * Replaces _apply_m2l + _accumulate_m2l_* (runtime/kernels/_m2l.py:1439-1793) and
+ * m2l_real_fused_pallas (pallas/m2l_real_fused.py:493-573). */
+int32_t jfm_m2l(void* stream, int32_t dtype, int32_t p, const int32_t* targets, int32_t num_targets,
+                const int32_t* far_off, const int32_t* far_src, const void* multipoles,
+                const void* centers4, void* locals);
+
+/* One L2L level: locals[c] += l2l(locals[parent[c]], c_parent - c_c) for c in nodes[count].
+ * Replaces _propagate_solidfmm_locals_by_level (runtime/kernels/_l2l.py:210-409). */
+int32_t jfm_l2l_level(void* stream, int32_t dtype, int32_t p, const int32_t* nodes, int32_t count,
+                      const int32_t* parent, const void* centers4, void* locals);
+
+/* L2P far field only: writes rows out_index[i] of acc [rows][3] (and pot) with
+ * a = -G grad_delta phi, pot = -G phi, delta = centre - x, for every tree slot i.
+ * Replaces _evaluate_local_expansions_for_particles (runtime/kernels/_evaluate.py:2106-2485)
+ * and evaluate_local_real_with_grad (operators/real_p2m_l2p.py:494-547). */
+int32_t jfm_l2p(void* stream, int32_t dtype, int32_t p, const void* pos4, int32_t n, int32_t leaf_size,
+                int32_t num_internal, const void* locals, const void* centers4, double G, const int32_t* out_index,
+                void* acc, void* pot);
+
+/* P2P near field only (self leaf i != j plus near leaves), leaf_size <= 256.
+ * flags: JFM_EVAL_POTENTIAL, JFM_EVAL_COMPENSATED.  accumulate != 0 adds into
+ * the output rows instead of overwriting them.  Replaces
+ * compute_leaf_p2p_accelerations (nearfield/near_field.py:916) and
+ * nearfield_fused_leaf_pallas (pallas/nearfield_fused_leaf.py:288-510). */
+int32_t jfm_p2p(void* stream, int32_t dtype, const void* pos4, int32_t n, int32_t num_leaves, int32_t leaf_size,
+                const int32_t* near_off, const int32_t* near_src, double eps2, double G, const int32_t* out_index,
+                void* acc, void* pot, int32_t flags, int32_t accumulate);
+
+/* Fused leaf evaluation: P2P near field (self leaf i != j plus near leaves) and
+ * L2P far field (a = -G grad_delta phi, delta = centre - x), per flags.  Results
+ * for tree slot i go to row out_index[i] of acc [rows][3] (and pot [rows]);
+ * out_index NULL means identity.  Replaces compute_leaf_p2p_accelerations
+ * (nearfield/near_field.py:916), nearfield_fused_leaf_pallas
+ * (pallas/nearfield_fused_leaf.py:288-510) and _evaluate_local_expansions_for_particles
+ * (runtime/kernels/_evaluate.py:2106-2485) with the combine/un-permute of
+ * _evaluate.py:1155-1170 and fmm_evaluate.py:531. */
+int32_t jfm_evaluate(void* stream, int32_t dtype, int32_t p, const void* pos4, int32_t n, int32_t num_leaves,
+                     int32_t leaf_size, int32_t num_internal, const int32_t* near_off, const int32_t* near_src,
+                     const void* locals, const void* centers4, double eps2, double G,
+                     const int32_t* out_index, void* acc, void* pot, int32_t flags);
+
+/* Float64 direct sum at targets[nt] (tree slots) over all n particles of pos4
+ * (float64), self excluded; writes acc [nt][3] and pot [nt] (pot may be NULL).
+ * scratch must hold jfm_direct_scratch_doubles(nt) doubles.
+ * Replaces direct_sum_gravitational_acceleration (jaccpot/autodiff.py:12-98). */
+int64_t jfm_direct_scratch_doubles(int32_t nt);
+int32_t jfm_direct_sum_f64(void* stream, const double* pos4, int32_t n, const int32_t* targets, int32_t nt,
+                           double eps2, double G, double* scratch, double* acc, double* pot);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JFM_H */

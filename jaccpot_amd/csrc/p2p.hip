@@ -1,0 +1,252 @@
+// Near-field P2P: softened Newtonian sums over the self leaf and the near-list
+// leaves of every target leaf, plus the fused evaluate entry point
+// (L2P far field + P2P near field + combine + un-permute).
+//
+// Mapping: one 256-thread workgroup per target leaf.  Thread t owns target
+// t % WT of the leaf (WT = leaf width rounded up to a power of two) and source
+// partition t / WT.  The leaf's source slots (self leaf first, then the near
+// leaves, each leaf_size slots wide) stream through LDS in tiles; inside a
+// tile partition q takes slots q, q + NPART, ... so every lane of a wave reads
+// one broadcast address per interaction and the tail tile stays balanced.
+// Interactions use the hardware v_rsq_f32 (float32); per-tile partial sums are
+// folded into the running total with Kahan compensation when requested; the
+// partitions are reduced through LDS in fixed order (deterministic).
+
+#include <hip/hip_runtime.h>
+
+#include "device_math.hpp"
+#include "launch.hpp"
+#include "rows.hpp"
+
+namespace jfm {
+namespace {
+
+constexpr int kThreads = 256;
+
+template <typename T>
+__device__ __forceinline__ T fast_rsqrt(T x) {
+  if constexpr (sizeof(T) == 4)
+    return __builtin_amdgcn_rsqf(x);
+  else
+    return 1.0 / sqrt(x);
+}
+
+template <typename T>
+__device__ __forceinline__ void kahan_add(T& s, T& c, T v) {
+  const T y = v - c;
+  const T t = s + y;
+  c = (t - s) - y;
+  s = t;
+}
+
+// SAFE: guard r^2 == 0 (unsoftened self term / coincident points).
+template <typename T, int WT, bool POT, bool COMP, bool SAFE>
+__global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos4, int32_t n, int32_t leaf_size,
+                                                       const int32_t* __restrict__ near_off,
+                                                       const int32_t* __restrict__ near_src, T eps2, T G,
+                                                       const int32_t* __restrict__ out_index, T* __restrict__ acc,
+                                                       T* __restrict__ pot, int32_t accumulate) {
+  constexpr int NPART = kThreads / WT;
+  constexpr int TILE = sizeof(T) == 4 ? 1024 : 512;
+  constexpr int LOADS = TILE / kThreads;
+  static_assert(2 * kThreads <= TILE, "reduction buffer must fit in the tile");
+  __shared__ V4<T> tile[TILE];
+
+  const int leaf = blockIdx.x;
+  const int i = threadIdx.x % WT;
+  const int part = threadIdx.x / WT;
+  const int first = leaf * leaf_size;
+  const int cnt = min(leaf_size, n - first);
+  const V4<T> xi = load4(pos4, first + (i < cnt ? i : 0));
+  const int nb = near_off[leaf];
+  const int nsl = 1 + (near_off[leaf + 1] - nb);
+  const int total = nsl * leaf_size;
+
+  T sx = 0, sy = 0, sz = 0, sp = 0;
+  T kx = 0, ky = 0, kz = 0, kp = 0;
+  for (int base = 0; base < total; base += TILE) {
+    __syncthreads();
+#pragma unroll
+    for (int l = 0; l < LOADS; ++l) {
+      const int slot = base + l * kThreads + threadIdx.x;
+      V4<T> q{T(0), T(0), T(0), T(0)};
+      if (slot < total) {
+        const int qi = slot / leaf_size;
+        const int o = slot - qi * leaf_size;
+        const int sl = qi == 0 ? leaf : near_src[nb + qi - 1];
+        const int g = sl * leaf_size + o;
+        if (g < n) q = load4(pos4, g);
+      }
+      tile[l * kThreads + threadIdx.x] = q;
+    }
+    __syncthreads();
+    const int lim = min(TILE, total - base);
+    T tx = 0, ty = 0, tz = 0, tp = 0;
+#pragma unroll 4
+    for (int j = part; j < lim; j += NPART) {
+      const V4<T> s = tile[j];
+      const T dx = s.x - xi.x;
+      const T dy = s.y - xi.y;
+      const T dz = s.z - xi.z;
+      const T r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, eps2)));
+      T rinv = fast_rsqrt(r2);
+      if (SAFE) rinv = r2 > T(0) ? rinv : T(0);
+      const T mr = s.w * rinv;
+      const T mr3 = mr * (rinv * rinv);
+      tx = fma(mr3, dx, tx);
+      ty = fma(mr3, dy, ty);
+      tz = fma(mr3, dz, tz);
+      if (POT) tp += mr;
+    }
+    if (COMP) {
+      kahan_add(sx, kx, tx);
+      kahan_add(sy, ky, ty);
+      kahan_add(sz, kz, tz);
+      if (POT) kahan_add(sp, kp, tp);
+    } else {
+      sx += tx;
+      sy += ty;
+      sz += tz;
+      if (POT) sp += tp;
+    }
+  }
+  __syncthreads();
+  tile[threadIdx.x] = V4<T>{sx, sy, sz, sp};
+  tile[kThreads + threadIdx.x] = V4<T>{kx, ky, kz, kp};
+  __syncthreads();
+  if (part == 0 && i < cnt) {
+    T ax = 0, ay = 0, az = 0, ap = 0, cx = 0, cy = 0, cz = 0, cp = 0;
+#pragma unroll
+    for (int q = 0; q < NPART; ++q) {
+      const V4<T> v = tile[q * WT + i];
+      const V4<T> c = tile[kThreads + q * WT + i];
+      ax += v.x;
+      ay += v.y;
+      az += v.z;
+      ap += v.w;
+      cx += c.x;
+      cy += c.y;
+      cz += c.z;
+      cp += c.w;
+    }
+    ax -= cx;
+    ay -= cy;
+    az -= cz;
+    ap -= cp;
+    if (POT && eps2 > T(0)) ap -= xi.w * fast_rsqrt(eps2);  // remove the softened self term
+    const int64_t row = out_index ? out_index[first + i] : first + i;
+    if (accumulate) {
+      acc[3 * row + 0] += G * ax;
+      acc[3 * row + 1] += G * ay;
+      acc[3 * row + 2] += G * az;
+      if (POT) pot[row] += -G * ap;
+    } else {
+      acc[3 * row + 0] = G * ax;
+      acc[3 * row + 1] = G * ay;
+      acc[3 * row + 2] = G * az;
+      if (POT) pot[row] = -G * ap;
+    }
+  }
+}
+
+template <typename T, int WT>
+int32_t launch_p2p_wt(hipStream_t s, const T* pos4, int32_t n, int32_t leaf_size, int32_t num_leaves,
+                      const int32_t* near_off, const int32_t* near_src, T eps2, T G, const int32_t* out_index,
+                      T* acc, T* pot, bool comp, bool accumulate) {
+  const bool want_pot = pot != nullptr;
+  const bool safe = !(eps2 > T(0));
+#define JFM_P2P_LAUNCH(POT, COMP, SAFE)                                                                        \
+  p2p_kernel<T, WT, POT, COMP, SAFE><<<num_leaves, kThreads, 0, s>>>(pos4, n, leaf_size, near_off, near_src, \
+                                                                     eps2, G, out_index, acc, pot,           \
+                                                                     accumulate ? 1 : 0)
+  if (want_pot) {
+    if (comp) {
+      if (safe) JFM_P2P_LAUNCH(true, true, true); else JFM_P2P_LAUNCH(true, true, false);
+    } else {
+      if (safe) JFM_P2P_LAUNCH(true, false, true); else JFM_P2P_LAUNCH(true, false, false);
+    }
+  } else {
+    if (comp) {
+      if (safe) JFM_P2P_LAUNCH(false, true, true); else JFM_P2P_LAUNCH(false, true, false);
+    } else {
+      if (safe) JFM_P2P_LAUNCH(false, false, true); else JFM_P2P_LAUNCH(false, false, false);
+    }
+  }
+#undef JFM_P2P_LAUNCH
+  return check_launch("jfm_p2p");
+}
+
+template <typename T>
+int32_t launch_p2p(hipStream_t s, const T* pos4, int32_t n, int32_t leaf_size, int32_t num_leaves,
+                   const int32_t* near_off, const int32_t* near_src, T eps2, T G, const int32_t* out_index, T* acc,
+                   T* pot, bool comp, bool accumulate) {
+  if (leaf_size <= 16)
+    return launch_p2p_wt<T, 16>(s, pos4, n, leaf_size, num_leaves, near_off, near_src, eps2, G, out_index, acc,
+                                pot, comp, accumulate);
+  if (leaf_size <= 32)
+    return launch_p2p_wt<T, 32>(s, pos4, n, leaf_size, num_leaves, near_off, near_src, eps2, G, out_index, acc,
+                                pot, comp, accumulate);
+  if (leaf_size <= 64)
+    return launch_p2p_wt<T, 64>(s, pos4, n, leaf_size, num_leaves, near_off, near_src, eps2, G, out_index, acc,
+                                pot, comp, accumulate);
+  if (leaf_size <= 128)
+    return launch_p2p_wt<T, 128>(s, pos4, n, leaf_size, num_leaves, near_off, near_src, eps2, G, out_index, acc,
+                                 pot, comp, accumulate);
+  return launch_p2p_wt<T, 256>(s, pos4, n, leaf_size, num_leaves, near_off, near_src, eps2, G, out_index, acc,
+                               pot, comp, accumulate);
+}
+
+}  // namespace
+}  // namespace jfm
+
+using namespace jfm;
+
+extern "C" {
+
+int32_t jfm_l2p(void* stream, int32_t dtype, int32_t p, const void* pos4, int32_t n, int32_t leaf_size,
+                int32_t num_internal, const void* locals, const void* centers4, double G, const int32_t* out_index,
+                void* acc, void* pot);
+
+int32_t jfm_p2p(void* stream, int32_t dtype, const void* pos4, int32_t n, int32_t num_leaves, int32_t leaf_size,
+                const int32_t* near_off, const int32_t* near_src, double eps2, double G, const int32_t* out_index,
+                void* acc, void* pot, int32_t flags, int32_t accumulate) {
+  if (n < 1 || leaf_size < 1 || leaf_size > 256 || num_leaves != (n + leaf_size - 1) / leaf_size)
+    return fail(JFM_ERR_ARG, "jfm_p2p: inconsistent sizes (leaf_size must be 1..256)");
+  if (!pos4 || !near_off || !near_src || !acc) return fail(JFM_ERR_ARG, "jfm_p2p: NULL buffer");
+  if (!(eps2 >= 0.0)) return fail(JFM_ERR_ARG, "jfm_p2p: eps2 must be >= 0");
+  hipStream_t s = as_stream(stream);
+  const bool comp = (flags & JFM_EVAL_COMPENSATED) != 0;
+  void* potp = (flags & JFM_EVAL_POTENTIAL) ? pot : nullptr;
+  if ((flags & JFM_EVAL_POTENTIAL) && !pot) return fail(JFM_ERR_ARG, "jfm_p2p: potential requested, pot is NULL");
+  if (dtype == JFM_F32)
+    return launch_p2p<float>(s, (const float*)pos4, n, leaf_size, num_leaves, near_off, near_src,
+                             static_cast<float>(eps2), static_cast<float>(G), out_index, (float*)acc, (float*)potp,
+                             comp, accumulate != 0);
+  if (dtype == JFM_F64)
+    return launch_p2p<double>(s, (const double*)pos4, n, leaf_size, num_leaves, near_off, near_src, eps2, G,
+                              out_index, (double*)acc, (double*)potp, comp, accumulate != 0);
+  return fail(JFM_ERR_ARG, "jfm_p2p: bad dtype");
+}
+
+int32_t jfm_evaluate(void* stream, int32_t dtype, int32_t p, const void* pos4, int32_t n, int32_t num_leaves,
+                     int32_t leaf_size, int32_t num_internal, const int32_t* near_off, const int32_t* near_src,
+                     const void* locals, const void* centers4, double eps2, double G, const int32_t* out_index,
+                     void* acc, void* pot, int32_t flags) {
+  const bool near = (flags & JFM_EVAL_NEAR) != 0;
+  const bool far = (flags & JFM_EVAL_FAR) != 0;
+  if (!near && !far) return fail(JFM_ERR_ARG, "jfm_evaluate: neither JFM_EVAL_NEAR nor JFM_EVAL_FAR set");
+  if (num_internal != num_leaves - 1) return fail(JFM_ERR_ARG, "jfm_evaluate: num_internal must be L-1");
+  if ((flags & JFM_EVAL_POTENTIAL) && !pot) return fail(JFM_ERR_ARG, "jfm_evaluate: pot is NULL");
+  void* potp = (flags & JFM_EVAL_POTENTIAL) ? pot : nullptr;
+  if (far) {
+    int32_t rc = jfm_l2p(stream, dtype, p, pos4, n, leaf_size, num_internal, locals, centers4, G, out_index, acc,
+                         potp);
+    if (rc != JFM_OK) return rc;
+  }
+  if (near)
+    return jfm_p2p(stream, dtype, pos4, n, num_leaves, leaf_size, near_off, near_src, eps2, G, out_index, acc, potp,
+                   flags, far ? 1 : 0);
+  return JFM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,37 @@
+"""Seeded synthetic inputs for the benchmark workloads (float64, numpy).
+
+* uniform(n, seed): U[-1,1]^3, masses U[0.5,1.5];
+* plummer(n, seed): Plummer sphere a=1 truncated at r=10a by the inverse CDF
+  r = (u^{-2/3} - 1)^{-1/2}, u ~ U(0, M(10)), M(r) = r^3/(r^2+1)^{3/2},
+  isotropic directions, equal masses 1/N;
+* blobs(n, seed): 64 Gaussian blobs sigma=0.05, centres U[-1,1]^3, masses
+  lognormal(0, 0.5).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+
+def uniform(n: int, seed: int):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(-1.0, 1.0, size=(n, 3)), rng.uniform(0.5, 1.5, size=n)
+
+
+def plummer(n: int, seed: int, a: float = 1.0, rmax: float = 10.0):
+    rng = np.random.default_rng(seed)
+    mmax = rmax ** 3 / (rmax ** 2 + 1.0) ** 1.5
+    u = rng.uniform(0.0, mmax, size=n)
+    r = a / np.sqrt(u ** (-2.0 / 3.0) - 1.0)
+    cth = rng.uniform(-1.0, 1.0, size=n)
+    phi = rng.uniform(0.0, 2.0 * np.pi, size=n)
+    sth = np.sqrt(1.0 - cth * cth)
+    x = np.stack([r * sth * np.cos(phi), r * sth * np.sin(phi), r * cth], axis=1)
+    return x, np.full(n, 1.0 / n)
+
+
+def blobs(n: int, seed: int, nblobs: int = 64, sigma: float = 0.05):
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(-1.0, 1.0, size=(nblobs, 3))
+    x = centres[np.arange(n) % nblobs] + sigma * rng.standard_normal(size=(n, 3))
+    return x, rng.lognormal(0.0, 0.5, size=n)

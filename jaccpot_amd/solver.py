@@ -1,0 +1,328 @@
+"""``FastMultipoleMethod`` -- the reference's public facade, on libjfm.
+
+Mirrors jaccpot/solver.py:540-1356 for the fixed-topology, real-basis path:
+
+* ``prepare_state(positions, masses, *, bounds, leaf_size, max_order, theta)``
+  (solver.py:1012-1084): host topology (tree + far/near lists), then on the
+  GPU the gather into tree order, leaf P2M, M2M by level, M2L over the far
+  list and the L2L cascade; returns an :class:`FMMPreparedState`.
+* ``evaluate_prepared_state(state, *, target_indices, return_potential)``
+  (solver.py:1292-1356): fused L2P + P2P on the GPU, results in input order.
+* ``compute_accelerations`` = prepare + evaluate (fmm_evaluate.py:289-322).
+
+Errors follow the reference: ``ValueError`` for bad configuration
+(solver.py:709-722), ``NotImplementedError`` for unsupported combinations.
+Every numeric step runs in libjfm's HIP kernels; when the library or a GPU is
+missing the constructor raises instead of falling back.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Any, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from .topology import Topology, build_topology
+
+_PRESETS = ("fast", "balanced", "accurate", "large_n_gpu")
+
+
+def _require_gpu() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("jaccpot_amd needs a ROCm GPU: torch.cuda.is_available() is False")
+    nat.load()
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+@dataclass
+class FMMPreparedState:
+    """Frozen topology + device buffers of one prepared evaluation.
+
+    Counterpart of FMMPreparedState (runtime/fmm_state.py:802-1073): host
+    topology, tree-order particles, node centres, multipoles and locals.
+    Host-built and not traceable, like the reference's (solver.py:1068-1070).
+    """
+
+    topology: Topology
+    order: int
+    dtype: torch.dtype
+    G: float
+    softening: float
+    theta: float
+    device: torch.device
+    input_kind: str  # "numpy" or "torch"
+    input_dtype: Any
+    input_device: Any
+    cs: int
+    perm: torch.Tensor
+    pos4: torch.Tensor
+    node_range: torch.Tensor
+    left: torch.Tensor
+    right: torch.Tensor
+    parent: torch.Tensor
+    m2m_nodes: torch.Tensor
+    l2l_nodes: torch.Tensor
+    m2l_targets: torch.Tensor
+    far_off: torch.Tensor
+    far_src: torch.Tensor
+    near_off: torch.Tensor
+    near_src: torch.Tensor
+    centers4: torch.Tensor
+    multipoles: torch.Tensor
+    locals: torch.Tensor
+    compensated: bool = True
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def num_particles(self) -> int:
+        return self.topology.num_particles
+
+
+class FastMultipoleMethod:
+    """MI355X-native FMM facade with the reference's constructor surface.
+
+    Parameters mirror jaccpot/solver.py:664-692.  Only the real basis and the
+    fixed radix topology are implemented; ``m2l_impl`` accepts ``None`` or
+    ``"rot_scale"`` (the reference's real-basis default, solver.py:723-726).
+    ``precision`` picks float32 or float64 kernels; ``None`` follows the input
+    dtype.  ``compensated`` (default on) selects blocked Kahan accumulation in
+    the float32 P2P sums.
+    """
+
+    def __init__(
+        self,
+        *,
+        preset: str = "fast",
+        basis: str = "real",
+        m2l_impl: Optional[str] = None,
+        adaptive_order: bool = False,
+        p_gears: Optional[Sequence[int]] = None,
+        theta: float = 0.6,
+        G: float = 1.0,
+        softening: float = 1e-3,
+        precision: Optional[str] = None,
+        working_dtype: Any = None,
+        compensated: bool = True,
+        num_threads: int = 0,
+        **legacy_kwargs: Any,
+    ):
+        preset_norm = str(getattr(preset, "value", preset)).strip().lower()
+        if preset_norm not in _PRESETS:
+            raise ValueError(f"unknown preset {preset!r}; expected one of {_PRESETS}")
+        if str(basis).strip().lower() != "real":
+            raise NotImplementedError("only basis='real' (the Dehnen real basis) is implemented")
+        if m2l_impl not in (None, "rot_scale"):
+            raise NotImplementedError(f"m2l_impl={m2l_impl!r} is not implemented (use 'rot_scale')")
+        if adaptive_order or p_gears is not None:
+            raise NotImplementedError("adaptive / per-pair M2L orders are not available in this build")
+        if legacy_kwargs:
+            raise TypeError(f"unsupported keyword arguments: {sorted(legacy_kwargs)}")
+        if not (theta > 0.0):
+            raise ValueError("theta must be > 0")
+        if not (softening >= 0.0):
+            raise ValueError("softening must be >= 0")
+        dtype = None
+        if precision is not None:
+            p = str(precision).strip().lower()
+            if p not in ("fp32", "fp64"):
+                raise ValueError("precision must be one of ('fp32', 'fp64')")
+            dtype = torch.float32 if p == "fp32" else torch.float64
+        if working_dtype is not None:
+            wd = torch.float64 if np.dtype(str(working_dtype).replace("torch.", "")) == np.float64 else torch.float32
+            if dtype is not None and wd != dtype:
+                raise ValueError("precision conflicts with explicit working_dtype; use only one or ensure they match")
+            dtype = wd
+        if preset_norm == "large_n_gpu" and dtype is None:
+            dtype = torch.float32
+        self.preset = preset_norm
+        self.basis = "real"
+        self.theta = float(theta)
+        self.G = float(G)
+        self.softening = float(softening)
+        self.working_dtype = dtype
+        self.compensated = bool(compensated)
+        self.num_threads = int(num_threads)
+        self.device = _require_gpu()
+        self._last_state: Optional[FMMPreparedState] = None
+
+    # ------------------------------------------------------------------ utils
+    def _as_inputs(self, positions, masses):
+        if isinstance(positions, torch.Tensor):
+            kind, in_dtype, in_dev = "torch", positions.dtype, positions.device
+            x_host = positions.detach().to("cpu", torch.float64).numpy()
+            m_host = torch.as_tensor(masses).detach().to("cpu", torch.float64).numpy()
+        else:
+            x_np = np.asarray(positions)
+            kind, in_dtype, in_dev = "numpy", x_np.dtype, None
+            x_host = np.asarray(x_np, np.float64)
+            m_host = np.asarray(masses, np.float64)
+        if x_host.ndim != 2 or x_host.shape[1] != 3:
+            raise ValueError("positions must have shape (N, 3)")
+        if m_host.shape != (x_host.shape[0],):
+            raise ValueError("masses must have shape (N,) matching positions")
+        if x_host.shape[0] < 1:
+            raise ValueError("need at least one particle")
+        return x_host, m_host, kind, in_dtype, in_dev
+
+    def _work_dtype(self, in_dtype) -> torch.dtype:
+        if self.working_dtype is not None:
+            return self.working_dtype
+        if in_dtype in (torch.float64, np.float64) or str(in_dtype) in ("float64", "torch.float64"):
+            return torch.float64
+        return torch.float32
+
+    # -------------------------------------------------------------- prepare
+    def prepare_state(self, positions, masses, *, bounds=None, leaf_size: int = 16, max_order: int = 4,
+                      theta: Optional[float] = None) -> FMMPreparedState:
+        """Build the topology and run the upward + downward passes (solver.py:1012)."""
+        p = int(max_order)
+        if p < 1 or p > nat.MAX_ORDER:
+            raise ValueError(f"max_order must be in [1, {nat.MAX_ORDER}]")
+        if int(leaf_size) < 1 or int(leaf_size) > 256:
+            raise ValueError("leaf_size must be in [1, 256]")
+        th = self.theta if theta is None else float(theta)
+        if not (th > 0.0):
+            raise ValueError("theta must be > 0")
+        x_host, m_host, kind, in_dtype, in_dev = self._as_inputs(positions, masses)
+        if not np.all(np.isfinite(x_host)) or not np.all(np.isfinite(m_host)):
+            raise ValueError("positions and masses must be finite")
+        topo = build_topology(x_host, m_host, leaf_size=int(leaf_size), theta=th, bounds=bounds,
+                              num_threads=self.num_threads)
+        dtype = self._work_dtype(in_dtype)
+        dev = self.device
+        code = nat.dtype_code(dtype)
+        cs = nat.coeff_stride(p, dtype)
+
+        def i32(a):
+            return torch.as_tensor(np.ascontiguousarray(a, np.int32)).to(dev)
+
+        nodes = topo.num_nodes
+        state = FMMPreparedState(
+            topology=topo, order=p, dtype=dtype, G=self.G, softening=self.softening, theta=th, device=dev,
+            input_kind=kind, input_dtype=in_dtype, input_device=in_dev, cs=cs,
+            perm=i32(topo.perm),
+            pos4=torch.empty((topo.num_particles, 4), dtype=dtype, device=dev),
+            node_range=i32(topo.node_range), left=i32(topo.left if topo.left.size else np.zeros(1)),
+            right=i32(topo.right if topo.right.size else np.zeros(1)), parent=i32(topo.parent),
+            m2m_nodes=i32(topo.m2m_nodes if topo.m2m_nodes.size else np.zeros(1)),
+            l2l_nodes=i32(topo.l2l_nodes if topo.l2l_nodes.size else np.zeros(1)),
+            m2l_targets=i32(topo.m2l_targets if topo.m2l_targets.size else np.zeros(1)),
+            far_off=i32(topo.far_off), far_src=i32(topo.far_src if topo.far_src.size else np.zeros(1)),
+            near_off=i32(topo.near_off), near_src=i32(topo.near_src if topo.near_src.size else np.zeros(1)),
+            centers4=torch.zeros((nodes, 4), dtype=dtype, device=dev),
+            multipoles=torch.zeros((nodes, cs), dtype=dtype, device=dev),
+            locals=torch.zeros((nodes, cs), dtype=dtype, device=dev),
+            compensated=self.compensated,
+        )
+        xd = torch.as_tensor(x_host, device=dev)
+        md = torch.as_tensor(m_host, device=dev)
+        self.refresh_prepared_state(state, xd, md)
+        self._last_state = state
+        return state
+
+    def refresh_prepared_state(self, state: FMMPreparedState, positions: torch.Tensor, masses: torch.Tensor) -> None:
+        """Re-run gather + upward + downward at the state's frozen topology.
+
+        ``positions``/``masses`` are device tensors in input order (float32 or
+        float64).  The fixed-topology re-evaluation the reference performs in
+        its differentiable path (fmm_evaluate.py:542-731).
+        """
+        topo = state.topology
+        s = nat.current_stream()
+        code = nat.dtype_code(state.dtype)
+        src_code = nat.dtype_code(positions.dtype)
+        positions = positions.contiguous()
+        masses = masses.to(positions.dtype).contiguous()
+        nat.call("jfm_gather_sorted", s, src_code, code, positions.data_ptr(), masses.data_ptr(),
+                 state.perm.data_ptr(), topo.num_particles, state.pos4.data_ptr())
+        self._upward(state, s, code)
+        self._downward(state, s, code)
+
+    def _upward(self, state: FMMPreparedState, s, code) -> None:
+        topo = state.topology
+        nat.call("jfm_p2m", s, code, state.order, state.pos4.data_ptr(), topo.num_particles, topo.num_leaves,
+                 topo.leaf_size, topo.num_internal, state.multipoles.data_ptr(), state.centers4.data_ptr())
+        off = topo.m2m_off
+        base = state.m2m_nodes.data_ptr()
+        for lev in range(len(off) - 1):
+            a, b = int(off[lev]), int(off[lev + 1])
+            if b > a:
+                nat.call("jfm_m2m_level", s, code, state.order, base + 4 * a, b - a, state.left.data_ptr(),
+                         state.right.data_ptr(), state.node_range.data_ptr(), state.multipoles.data_ptr(),
+                         state.centers4.data_ptr())
+
+    def _downward(self, state: FMMPreparedState, s, code) -> None:
+        topo = state.topology
+        state.locals.zero_()
+        if topo.m2l_targets.size:
+            nat.call("jfm_m2l", s, code, state.order, state.m2l_targets.data_ptr(), int(topo.m2l_targets.size),
+                     state.far_off.data_ptr(), state.far_src.data_ptr(), state.multipoles.data_ptr(),
+                     state.centers4.data_ptr(), state.locals.data_ptr())
+        off = topo.l2l_off
+        base = state.l2l_nodes.data_ptr()
+        for lev in range(len(off) - 1):
+            a, b = int(off[lev]), int(off[lev + 1])
+            if b > a:
+                nat.call("jfm_l2l_level", s, code, state.order, base + 4 * a, b - a, state.parent.data_ptr(),
+                         state.centers4.data_ptr(), state.locals.data_ptr())
+
+    # ------------------------------------------------------------- evaluate
+    def _evaluate_device(self, state: FMMPreparedState, acc: torch.Tensor, pot: Optional[torch.Tensor],
+                         flags: Optional[int] = None) -> None:
+        topo = state.topology
+        if flags is None:
+            flags = nat.EVAL_NEAR | nat.EVAL_FAR
+            if state.compensated:
+                flags |= nat.EVAL_COMPENSATED
+        if pot is not None:
+            flags |= nat.EVAL_POTENTIAL
+        nat.call("jfm_evaluate", nat.current_stream(), nat.dtype_code(state.dtype), state.order,
+                 state.pos4.data_ptr(), topo.num_particles, topo.num_leaves, topo.leaf_size, topo.num_internal,
+                 state.near_off.data_ptr(), state.near_src.data_ptr(), state.locals.data_ptr(),
+                 state.centers4.data_ptr(), state.softening ** 2, state.G, state.perm.data_ptr(), acc.data_ptr(),
+                 pot.data_ptr() if pot is not None else None, flags)
+
+    def evaluate_prepared_state(self, state: FMMPreparedState, *, target_indices=None,
+                                return_potential: bool = False, jit_traversal=None,
+                                max_acc_derivative_order: int = 0):
+        """Accelerations (and potentials) in input order (solver.py:1292)."""
+        if max_acc_derivative_order != 0:
+            raise NotImplementedError("acceleration derivatives need the solidfmm basis, which is not built")
+        n = state.num_particles
+        acc = torch.empty((n, 3), dtype=state.dtype, device=state.device)
+        pot = torch.empty((n,), dtype=state.dtype, device=state.device) if return_potential else None
+        self._evaluate_device(state, acc, pot)
+        if target_indices is not None:
+            idx = torch.as_tensor(np.asarray(target_indices) if not isinstance(target_indices, torch.Tensor)
+                                  else target_indices, device=state.device).long()
+            if idx.ndim != 1:
+                raise ValueError("target_indices must be 1-D")
+            if idx.numel() and (int(idx.min()) < 0 or int(idx.max()) >= n):
+                raise ValueError("target_indices out of range")
+            acc = acc.index_select(0, idx)
+            if pot is not None:
+                pot = pot.index_select(0, idx)
+        acc = self._to_output(state, acc)
+        if return_potential:
+            return acc, self._to_output(state, pot)
+        return acc
+
+    def _to_output(self, state: FMMPreparedState, t: torch.Tensor):
+        if state.input_kind == "numpy":
+            out_dtype = np.dtype(state.input_dtype) if np.issubdtype(np.dtype(state.input_dtype), np.floating) \
+                else np.float64
+            return t.detach().cpu().numpy().astype(out_dtype, copy=False)
+        out_dtype = state.input_dtype if state.input_dtype in (torch.float32, torch.float64) else t.dtype
+        return t.to(device=state.input_device, dtype=out_dtype)
+
+    def compute_accelerations(self, positions, masses, *, target_indices=None, bounds=None, leaf_size: int = 16,
+                              max_order: int = 4, return_potential: bool = False, theta: Optional[float] = None,
+                              reuse_prepared_state: bool = False, max_acc_derivative_order: int = 0):
+        """prepare_state + evaluate_prepared_state (solver.py:917)."""
+        state = self.prepare_state(positions, masses, bounds=bounds, leaf_size=leaf_size, max_order=max_order,
+                                   theta=theta)
+        return self.evaluate_prepared_state(state, target_indices=target_indices, return_potential=return_potential,
+                                            max_acc_derivative_order=max_acc_derivative_order)

@@ -1,0 +1,174 @@
+"""GPU parity: every HIP kernel against the float64 oracle on the same inputs.
+
+Float64 kernels must match the oracle to 1e-12 (relative); float32 kernels to
+1e-5 (relative L2 over all particles, plus a per-particle median bound).  The
+topology is built independently on both sides and must agree bit-exactly (the
+CPU suite checks that too), so differences here are kernel arithmetic only.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import fmm as ofmm
+from oracle import harmonics as H
+from oracle import ics
+from oracle import topology as OT
+from oracle.direct import direct_sum, relative_errors
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _fmm(precision, theta, eps, compensated=True):
+    from jaccpot_amd import FastMultipoleMethod
+
+    return FastMultipoleMethod(precision=precision, theta=theta, softening=eps, compensated=compensated)
+
+
+def _rel_rows(a, b):
+    """max over rows of |a-b| / max|b| (row scale), robust to zero rows."""
+    scale = np.maximum(np.abs(b).max(axis=1, keepdims=True), 1e-300)
+    return float((np.abs(a - b) / scale).max())
+
+
+CASES64 = [
+    # (name, generator, n, leaf, p, theta, eps)
+    ("uniform4096_leaf16_p4_eps0", "uniform", 4096, 16, 4, 0.6, 0.0),
+    ("uniform1500_leaf8_p6_eps1e-2", "uniform", 1500, 8, 6, 0.5, 1e-2),
+    ("plummer3000_leaf32_p3", "plummer", 3000, 32, 3, 0.7, 1e-3),
+    ("blobs2000_leaf64_p8", "blobs", 2000, 64, 8, 0.6, 1e-3),
+    ("ragged7_leaf4_p2", "uniform", 7, 4, 2, 0.6, 0.0),
+    ("single_particle", "uniform", 1, 16, 4, 0.6, 0.0),
+    ("uniform777_leaf5_p5", "uniform", 777, 5, 5, 0.55, 1e-3),
+]
+
+
+def _inputs(gen, n, seed=0):
+    return getattr(ics, gen)(n, seed)
+
+
+@pytest.mark.parametrize("case", CASES64, ids=[c[0] for c in CASES64])
+def test_fp64_pipeline_matches_oracle(case, require_gpu):
+    name, gen, n, leaf, p, theta, eps = case
+    print(f"[gpu] fp64 pipeline {name}", flush=True)
+    x, m = _inputs(gen, n)
+    fmm = _fmm("fp64", theta, eps)
+    state = fmm.prepare_state(x, m, leaf_size=leaf, max_order=p)
+    acc, pot = fmm.evaluate_prepared_state(state, return_potential=True)
+    torch.cuda.synchronize()
+
+    topo = OT.build_topology(x, m, leaf, theta)
+    assert np.array_equal(topo.perm, state.topology.perm)
+    assert np.array_equal(topo.far_src, state.topology.far_src)
+    xs, ms = x[topo.perm], m[topo.perm]
+    C = H.sh_size(p)
+    M = ofmm.upward(topo, xs, ms, p)
+    Lc = ofmm.downward(topo, M, p)
+    acc_s, pot_s = ofmm.evaluate(topo, xs, ms, Lc, p, 1.0, eps)
+
+    centers = state.centers4.cpu().numpy()
+    assert np.max(np.abs(centers[:, :3] - topo.center)) <= 1e-13 * max(1.0, np.abs(topo.center).max())
+    np.testing.assert_allclose(centers[:, 3], topo.mass, rtol=1e-13)
+    gM = state.multipoles.cpu().numpy()[:, :C]
+    assert _rel_rows(gM, M) < 1e-12
+    gL = state.locals.cpu().numpy()[:, :C]
+    if np.abs(Lc).max() > 0:
+        assert _rel_rows(gL, Lc) < 1e-12
+    ref_acc = np.empty_like(acc_s)
+    ref_pot = np.empty_like(pot_s)
+    ref_acc[topo.perm] = acc_s
+    ref_pot[topo.perm] = pot_s
+    if n > 1:
+        rel = np.linalg.norm(acc - ref_acc) / np.linalg.norm(ref_acc)
+        assert rel < 1e-12, rel
+        assert np.median(relative_errors(acc, ref_acc)) < 1e-12
+        assert np.linalg.norm(pot - ref_pot) / np.linalg.norm(ref_pot) < 1e-12
+    else:
+        assert np.all(acc == 0.0) and np.all(pot == 0.0)
+
+
+CASES32 = [
+    ("uniform4096_leaf16_p4", "uniform", 4096, 16, 4, 0.6, 0.0),
+    ("plummer8192_leaf32_p4", "plummer", 8192, 32, 4, 0.6, 1e-3),
+    ("plummer8192_leaf64_p6", "plummer", 8192, 64, 6, 0.6, 1e-3),
+    ("blobs4096_leaf32_p4", "blobs", 4096, 32, 4, 0.6, 1e-3),
+]
+
+
+@pytest.mark.parametrize("case", CASES32, ids=[c[0] for c in CASES32])
+def test_fp32_pipeline_matches_oracle(case, require_gpu):
+    name, gen, n, leaf, p, theta, eps = case
+    print(f"[gpu] fp32 pipeline {name}", flush=True)
+    x, m = _inputs(gen, n, seed=1)
+    fmm = _fmm("fp32", theta, eps)
+    acc = fmm.compute_accelerations(x.astype(np.float32), m.astype(np.float32), leaf_size=leaf, max_order=p)
+    torch.cuda.synchronize()
+    ref = ofmm.fmm_accelerations(x.astype(np.float32).astype(np.float64), m.astype(np.float32).astype(np.float64),
+                                 leaf_size=leaf, p=p, theta=theta, eps=eps)
+    rel = np.linalg.norm(acc - ref) / np.linalg.norm(ref)
+    errs = relative_errors(acc, ref)
+    print(f"[gpu]   rel-L2 {rel:.3e} median {np.median(errs):.3e} max {errs.max():.3e}", flush=True)
+    assert rel < 1e-5
+    assert np.median(errs) < 1e-5
+
+
+def test_near_and_far_parts_separately(require_gpu):
+    """P2P-only and L2P-only evaluations against the oracle's split."""
+    from jaccpot_amd import _native as nat
+
+    x, m = ics.plummer(3000, 5)
+    p, leaf, theta, eps = 4, 16, 0.6, 1e-3
+    fmm = _fmm("fp64", theta, eps)
+    state = fmm.prepare_state(x, m, leaf_size=leaf, max_order=p)
+    n = x.shape[0]
+    out = {}
+    for name, flags in (("near", nat.EVAL_NEAR), ("far", nat.EVAL_FAR)):
+        acc = torch.empty((n, 3), dtype=torch.float64, device=state.device)
+        pot = torch.empty((n,), dtype=torch.float64, device=state.device)
+        fmm._evaluate_device(state, acc, pot, flags=flags)
+        out[name] = (acc.cpu().numpy(), pot.cpu().numpy())
+    topo = OT.build_topology(x, m, leaf, theta)
+    xs, ms = x[topo.perm], m[topo.perm]
+    M = ofmm.upward(topo, xs, ms, p)
+    Lc = ofmm.downward(topo, M, p)
+    zero = np.zeros_like(Lc)
+    near_s, near_pot_s = ofmm.evaluate(topo, xs, ms, zero, p, 1.0, eps)
+    full_s, full_pot_s = ofmm.evaluate(topo, xs, ms, Lc, p, 1.0, eps)
+    far_s, far_pot_s = full_s - near_s, full_pot_s - near_pot_s
+    for (g_acc, g_pot), r_acc, r_pot in ((out["near"], near_s, near_pot_s), (out["far"], far_s, far_pot_s)):
+        ra = np.empty_like(r_acc)
+        rp = np.empty_like(r_pot)
+        ra[topo.perm] = r_acc
+        rp[topo.perm] = r_pot
+        assert np.linalg.norm(g_acc - ra) / np.linalg.norm(ra) < 1e-12
+        assert np.linalg.norm(g_pot - rp) / np.linalg.norm(rp) < 1e-12
+
+
+def test_direct_sum_kernel_matches_oracle(require_gpu):
+    from jaccpot_amd import direct_sum_gravitational_acceleration
+
+    x, m = ics.blobs(3000, 7)
+    tgt = np.random.default_rng(12345).choice(3000, 257, replace=False)
+    for eps in (0.0, 1e-2):
+        a, phi = direct_sum_gravitational_acceleration(x, m, softening=eps, target_indices=tgt, return_potential=True)
+        ra, rp = direct_sum(x, m, eps=eps, targets=tgt, return_potential=True)
+        np.testing.assert_allclose(a.cpu().numpy(), ra, rtol=1e-12, atol=1e-12 * np.abs(ra).max())
+        np.testing.assert_allclose(phi.cpu().numpy(), rp, rtol=1e-12)
+
+
+def test_target_indices_and_input_types(require_gpu):
+    x, m = ics.uniform(2048, 9)
+    fmm = _fmm("fp64", 0.6, 1e-3)
+    full = fmm.compute_accelerations(x, m, leaf_size=16, max_order=4)
+    idx = np.array([5, 0, 2047, 5, 100])
+    sub, pot = fmm.compute_accelerations(x, m, leaf_size=16, max_order=4, target_indices=idx, return_potential=True)
+    np.testing.assert_array_equal(sub, full[idx])
+    assert sub.dtype == np.float64 and pot.shape == (5,)
+    xt = torch.as_tensor(x, device="cuda")
+    mt = torch.as_tensor(m, device="cuda")
+    at = fmm.compute_accelerations(xt, mt, leaf_size=16, max_order=4)
+    assert isinstance(at, torch.Tensor) and at.device.type == "cuda"
+    np.testing.assert_array_equal(at.cpu().numpy(), full)
