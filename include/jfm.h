@@ -177,6 +177,18 @@ int64_t jfm_direct_scratch_doubles(int32_t nt);
 int32_t jfm_direct_sum_f64(void* stream, const double* pos4, int32_t n, const int32_t* targets, int32_t nt,
                            double eps2, double G, double* scratch, double* acc, double* pot);
 
+/* ------------------------------------------------------------------------
+ * Validation only: host (CPU) instantiation of the kernels' per-pair operator
+ * math, one pair per call, for checking against the float64 oracle without a
+ * GPU.  No product path uses it; it has no counterpart in the reference.
+ *   op 0 m2m (in: multipole, delta = source - destination)
+ *   op 1 m2l (in: multipole, delta = target - source)
+ *   op 2 l2l (in: local,     delta = parent - child)
+ *   op 3 p2m (in[0]: mass,   delta = particle - centre)
+ *   op 4 l2p (in: local,     delta = centre - point; out = phi, dphi/ddelta[3])
+ * in/out hold (p+1)^2 doubles; dtype selects the arithmetic precision. */
+int32_t jfm_host_operator(int32_t op, int32_t p, int32_t dtype, const double* in, const double* delta, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -59,6 +59,7 @@ _SIGNATURES = {
                                c_void_p, c_void_p, c_void_p, c_double, c_double, c_void_p, c_void_p, c_void_p,
                                c_int32]),
     "jfm_direct_scratch_doubles": (c_int64, [c_int32]),
+    "jfm_host_operator": (c_int32, [c_int32, c_int32, c_int32, c_void_p, c_void_p, c_void_p]),
     "jfm_direct_sum_f64": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_double, c_double, c_void_p,
                                      c_void_p, c_void_p]),
 }

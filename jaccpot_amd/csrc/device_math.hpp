@@ -2,7 +2,8 @@
 //
 // All routines act on register arrays and are fully unrolled for a
 // compile-time order P; the B_U swap matrices are folded in as literals
-// (tables_gen.h) so the checkerboard zeros cost nothing.
+// (tables_gen.h) so the structural zeros of B_U (about half its entries)
+// cost nothing.
 //
 // Conventions (packed index n*n + n + m):
 //   U_n^m  = Re R_n^|m| (m >= 0), Im R_n^|m| (m < 0), with the regular solid
@@ -58,7 +59,7 @@ constexpr double kFact[] = {1.0,
                             355687428096000.0};
 
 template <typename T>
-__device__ __forceinline__ T dev_sqrt(T x) {
+__host__ __device__ __forceinline__ T dev_sqrt(T x) {
   return sqrt(x);
 }
 
@@ -69,7 +70,7 @@ struct Align {
   T cx[P + 1], sx[P + 1];  // ax
   T r;                     // |delta|
 
-  __device__ __forceinline__ void init(T x, T y, T z) {
+  __host__ __device__ __forceinline__ void init(T x, T y, T z) {
     const T rho2 = x * x + y * y;
     const T r2 = rho2 + z * z;
     T c1a = T(1), s1a = T(0), c1x = T(1), s1x = T(0);
@@ -101,7 +102,7 @@ struct Align {
 
 // blk <- Dz(+/-a) blk for one degree; c[m], s[m] = cos/sin(m a).
 template <typename T, int L, bool NEG>
-__device__ __forceinline__ void apply_dz(T* blk, const T* c, const T* s) {
+__host__ __device__ __forceinline__ void apply_dz(T* blk, const T* c, const T* s) {
 #pragma unroll
   for (int m = 1; m <= L; ++m) {
     const T vp = blk[L + m], vm = blk[L - m];
@@ -113,7 +114,7 @@ __device__ __forceinline__ void apply_dz(T* blk, const T* c, const T* s) {
 
 // blk <- B_U blk (TRANS: B_U^T blk) for one degree, literals folded in.
 template <typename T, int L, bool TRANS>
-__device__ __forceinline__ void apply_bu(T* blk) {
+__host__ __device__ __forceinline__ void apply_bu(T* blk) {
   constexpr int W = 2 * L + 1;
   constexpr int O = bu_off(L);
   T in[W];
@@ -124,7 +125,6 @@ __device__ __forceinline__ void apply_bu(T* blk) {
     T acc = T(0);
 #pragma unroll
     for (int j = 0; j < W; ++j) {
-      if (((i - j) & 1) != 0) continue;  // checkerboard: B_U[i][j] = 0 for odd i - j
       const double b = TRANS ? kBU[O + j * W + i] : kBU[O + i * W + j];
       if (b != 0.0) acc = fma(static_cast<T>(b), in[j], acc);
     }
@@ -135,7 +135,7 @@ __device__ __forceinline__ void apply_bu(T* blk) {
 enum class Rot { ToZMultipole, FromZMultipole, ToZLocal, FromZLocal };
 
 template <typename T, int P, int L, Rot R>
-__device__ __forceinline__ void rotate_block(T* v, const Align<T, P>& a) {
+__host__ __device__ __forceinline__ void rotate_block(T* v, const Align<T, P>& a) {
   T* blk = v + L * L;
   if constexpr (R == Rot::ToZMultipole) {  // B Dz(-ax) B Dz(az)
     apply_dz<T, L, false>(blk, a.ca, a.sa);
@@ -161,7 +161,7 @@ __device__ __forceinline__ void rotate_block(T* v, const Align<T, P>& a) {
 }
 
 template <typename T, int P, Rot R, int L = 0>
-__device__ __forceinline__ void rotate_all(T* v, const Align<T, P>& a) {
+__host__ __device__ __forceinline__ void rotate_all(T* v, const Align<T, P>& a) {
   if constexpr (L == 0) {
     // degree 0 is invariant under every rotation
     rotate_all<T, P, R, 1>(v, a);
@@ -173,7 +173,7 @@ __device__ __forceinline__ void rotate_all(T* v, const Align<T, P>& a) {
 
 // z-axis M2M shift, in place (n descending keeps the inputs intact).
 template <typename T, int P>
-__device__ __forceinline__ void zshift_m2m(T* v, T dz) {
+__host__ __device__ __forceinline__ void zshift_m2m(T* v, T dz) {
   T dk[P + 1];
   dk[0] = T(1);
 #pragma unroll
@@ -193,7 +193,7 @@ __device__ __forceinline__ void zshift_m2m(T* v, T dz) {
 
 // z-axis L2L shift, in place (n ascending keeps the inputs intact).
 template <typename T, int P>
-__device__ __forceinline__ void zshift_l2l(T* v, T dz) {
+__host__ __device__ __forceinline__ void zshift_l2l(T* v, T dz) {
   T dk[P + 1];
   dk[0] = T(1);
 #pragma unroll
@@ -212,7 +212,7 @@ __device__ __forceinline__ void zshift_l2l(T* v, T dz) {
 
 // z-axis M2L core: out = Zcore(r) * in (in: z-frame multipole, out: z-frame local).
 template <typename T, int P>
-__device__ __forceinline__ void zm2l(const T* in, T* out, T r) {
+__host__ __device__ __forceinline__ void zm2l(const T* in, T* out, T r) {
   const T rr = r > T(1e-30) ? r : T(1e-30);
   const T rinv = T(1) / rr;
   T rp[P + 2];
@@ -240,7 +240,7 @@ __device__ __forceinline__ void zm2l(const T* in, T* out, T r) {
 
 // Regular solid harmonics: re/im of R_n^m for 0 <= m <= n <= P (triangular index).
 template <typename T, int P>
-__device__ __forceinline__ void solid_harmonics(T x, T y, T z, T* re, T* im) {
+__host__ __device__ __forceinline__ void solid_harmonics(T x, T y, T z, T* re, T* im) {
   const T r2 = x * x + y * y + z * z;
   re[0] = T(1);
   im[0] = T(0);
@@ -269,7 +269,7 @@ __device__ __forceinline__ void solid_harmonics(T x, T y, T z, T* re, T* im) {
 
 // phi = sum F U(d) and grad_d phi, with F packed (stride-free pointer to C values).
 template <typename T, int P, typename FLoad>
-__device__ __forceinline__ void l2p_eval(FLoad F, T dx, T dy, T dz, T& phi, T& gx, T& gy, T& gz) {
+__host__ __device__ __forceinline__ void l2p_eval(FLoad F, T dx, T dy, T dz, T& phi, T& gx, T& gy, T& gz) {
   T re[ntri(P)], im[ntri(P)];
   solid_harmonics<T, P>(dx, dy, dz, re, im);
   T p = T(0), ax = T(0), ay = T(0), az = T(0);
