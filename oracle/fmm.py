@@ -67,13 +67,9 @@ def downward(topo: Topology, M: np.ndarray, p: int, orders=None):
                 sel = orders == g
                 cg = H.sh_size(int(g))
                 contrib[sel, :cg] = H.m2l(M[src[sel], :cg], delta[sel], int(g))
-        # per-target sums in ascending source order (deterministic segmented sum)
-        for t in np.nonzero(np.diff(topo.far_off))[0]:
-            a, b = topo.far_off[t], topo.far_off[t + 1]
-            acc = np.zeros(C)
-            for k in range(a, b):
-                acc = acc + contrib[k]
-            Lc[t] = acc
+        # per-target sums over each target's contiguous, source-ascending segment
+        nz = np.nonzero(np.diff(topo.far_off))[0]
+        Lc[nz] = np.add.reduceat(contrib, topo.far_off[nz], axis=0)
     for level in nodes_by_depth_asc(topo):
         par = topo.parent[level]
         Lc[level] = Lc[level] + H.l2l(Lc[par], c[par] - c[level], p)
