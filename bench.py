@@ -74,6 +74,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=16384)
     ap.add_argument("--error-targets", type=int, default=4096)
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="run one torch copy of this many MiB first (HBM counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -86,6 +88,16 @@ def main() -> int:
     from jaccpot_amd import FastMultipoleMethod, direct_sum_gravitational_acceleration
     from jaccpot_amd import _native as nat
     from jaccpot_amd import ics
+
+    if args.calibrate_copy_mib > 0:
+        nel = args.calibrate_copy_mib * (1 << 20) // 4
+        src = torch.ones(nel, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calibrate_copy_mib} MiB read + {args.calibrate_copy_mib} MiB written")
+        del src, dst
 
     gen, N, seed, p, leaf, theta, eps, precision = WORKLOADS[args.workload]
     log(f"workload {args.workload}: {gen} N={N} p={p} leaf={leaf} theta={theta} eps={eps} {precision}")

@@ -22,6 +22,13 @@ namespace jfm {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ int xcd_contiguous(int b, int nblk) {
+  const int q = nblk / kXcds, r = nblk % kXcds;
+  const int x = b % kXcds, slot = b / kXcds;
+  return x * q + min(x, r) + slot;
+}
 
 template <typename T>
 __device__ __forceinline__ T fast_rsqrt(T x) {
@@ -52,7 +59,11 @@ __global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos
   static_assert(2 * kThreads <= TILE, "reduction buffer must fit in the tile");
   __shared__ V4<T> tile[TILE];
 
-  const int leaf = blockIdx.x;
+  // XCD-aware order: workgroups are dealt round-robin to the XCDs, so hand
+  // each XCD a contiguous run of leaves (neighbouring leaves share most of
+  // their source leaves, which then stay in that XCD's L2).  Bijective:
+  // XCD x < r owns q+1 leaves, the others q, where nblk = 8q + r.
+  const int leaf = xcd_contiguous(blockIdx.x, gridDim.x);
   const int i = threadIdx.x % WT;
   const int part = threadIdx.x / WT;
   const int first = leaf * leaf_size;
@@ -80,11 +91,13 @@ __global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos
       tile[l * kThreads + threadIdx.x] = q;
     }
     __syncthreads();
-    const int lim = min(TILE, total - base);
+    // Slots past `total` hold zero-mass sources, so every partition runs the
+    // same (wave-uniform, scalar-controlled) trip count.
+    const int trips = (min(TILE, total - base) + NPART - 1) / NPART;
     T tx = 0, ty = 0, tz = 0, tp = 0;
-#pragma unroll 4
-    for (int j = part; j < lim; j += NPART) {
-      const V4<T> s = tile[j];
+#pragma unroll 8
+    for (int k = 0; k < trips; ++k) {
+      const V4<T> s = tile[part + k * NPART];
       const T dx = s.x - xi.x;
       const T dy = s.y - xi.y;
       const T dz = s.z - xi.z;
