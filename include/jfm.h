@@ -139,6 +139,25 @@ int32_t jfm_m2l(void* stream, int32_t dtype, int32_t p, const int32_t* targets, 
 int32_t jfm_l2l_level(void* stream, int32_t dtype, int32_t p, const int32_t* nodes, int32_t count,
                       const int32_t* parent, const void* centers4, void* locals);
 
+/* Mixed precision, top three levels (float32 arrays + float64 shadows).
+ * shadow[nodes] maps each internal node of depth <= 2 to its shadow row
+ * (0..6) and every other node to -1; mult64 [7][jfm_coeff_stride(p, JFM_F64)],
+ * centers64 [7][4].  jfm_m2m_top: M2M of the shadowed nodes, levels deepest
+ * first (nodes / lev_off[nlev+1] on the device, lev_off_host the same offsets
+ * on the host), in float64, writing shadows and rounded float32 rows.
+ * jfm_l2l_top: seeds the shadows from the float32 M2L rows of top_nodes[ntop],
+ * then L2L in float64 into every node of depth 1..3 (levels shallowest first),
+ * adding once-rounded results to the float32 rows below the shadows.  Call it
+ * after jfm_m2l and before the float32 jfm_l2l_level sweeps of depth >= 4.
+ * The float64 M2M/L2L of the top levels of jaccpot's large-N lane runs in
+ * its working precision; the split is this build's precision policy. */
+int32_t jfm_m2m_top(void* stream, int32_t p, const int32_t* nodes, const int32_t* lev_off, int32_t nlev,
+                    const int32_t* lev_off_host, const int32_t* left, const int32_t* right, const int32_t* node_range,
+                    const int32_t* shadow, void* multipoles, void* centers4, void* mult64, void* centers64);
+int32_t jfm_l2l_top(void* stream, int32_t p, const int32_t* nodes, const int32_t* lev_off, int32_t nlev,
+                    const int32_t* lev_off_host, const int32_t* top_nodes, int32_t ntop, const int32_t* parent,
+                    const int32_t* shadow, const void* centers4, const void* centers64, void* locals, void* loc64);
+
 /* L2P far field only: writes rows out_index[i] of acc [rows][3] (and pot) with
  * a = -G grad_delta phi, pot = -G phi, delta = centre - x, for every tree slot i.
  * Replaces _evaluate_local_expansions_for_particles (runtime/kernels/_evaluate.py:2106-2485)

@@ -211,6 +211,145 @@ __global__ __launch_bounds__(256) void l2p_kernel(const T* __restrict__ pos4, in
   if (pot) pot[row] = -G * phi;
 }
 
+// Mixed precision: the internal nodes of the top three levels (depth <= 2,
+// at most 7 nodes) keep float64 shadow rows.  One workgroup walks their M2M
+// levels deepest first in float64, reading float32 rows for the children
+// below, and writes both the shadow and the rounded float32 row.
+template <int P>
+__global__ __launch_bounds__(kWave) void m2m_top_kernel(const int32_t* __restrict__ nodes,
+                                                        const int32_t* __restrict__ lev_off, int32_t nlev,
+                                                        const int32_t* __restrict__ left,
+                                                        const int32_t* __restrict__ right,
+                                                        const int32_t* __restrict__ node_range,
+                                                        const int32_t* __restrict__ shadow, float* __restrict__ mult,
+                                                        int32_t cs, float* __restrict__ centers4,
+                                                        double* __restrict__ mult64, int32_t cs64,
+                                                        double* __restrict__ centers64) {
+  constexpr int C = ncoef(P);
+  for (int lev = 0; lev < nlev; ++lev) {
+    const int t = threadIdx.x;
+    const int count = lev_off[lev + 1] - lev_off[lev];
+    if (t < count) {
+      const int node = nodes[lev_off[lev] + t];
+      const int ch[2] = {left[node], right[node]};
+      V4<double> cc[2];
+      double vc[2][C];
+      for (int side = 0; side < 2; ++side) {
+        const int c = ch[side];
+        const int sh = shadow[c];
+        if (sh >= 0) {
+          cc[side] = load4(centers64, sh);
+          load_row<double, C>(mult64 + static_cast<int64_t>(sh) * cs64, vc[side]);
+        } else {
+          const V4<float> f = load4(centers4, c);
+          cc[side] = V4<double>{f.x, f.y, f.z, f.w};
+          float tmp[C];
+          load_row<float, C>(mult + static_cast<int64_t>(c) * cs, tmp);
+          for (int j = 0; j < C; ++j) vc[side][j] = tmp[j];
+        }
+      }
+      const double tot = cc[0].w + cc[1].w;
+      double cx, cy, cz;
+      if (tot > 0.0) {
+        cx = (cc[0].w * cc[0].x + cc[1].w * cc[1].x) / tot;
+        cy = (cc[0].w * cc[0].y + cc[1].w * cc[1].y) / tot;
+        cz = (cc[0].w * cc[0].z + cc[1].w * cc[1].z) / tot;
+      } else {
+        const double nl = node_range[2 * ch[0] + 1] - node_range[2 * ch[0]] + 1;
+        const double nr = node_range[2 * ch[1] + 1] - node_range[2 * ch[1]] + 1;
+        cx = (nl * cc[0].x + nr * cc[1].x) / (nl + nr);
+        cy = (nl * cc[0].y + nr * cc[1].y) / (nl + nr);
+        cz = (nl * cc[0].z + nr * cc[1].z) / (nl + nr);
+      }
+      double acc[C];
+      for (int j = 0; j < C; ++j) acc[j] = 0.0;
+      for (int side = 0; side < 2; ++side) {
+        Align<double, P> a;
+        a.init(cc[side].x - cx, cc[side].y - cy, cc[side].z - cz);
+        rotate_all<double, P, Rot::ToZMultipole>(vc[side], a);
+        zshift_m2m<double, P>(vc[side], a.r);
+        rotate_all<double, P, Rot::FromZMultipole>(vc[side], a);
+        for (int j = 0; j < C; ++j) acc[j] += vc[side][j];
+      }
+      const int sh = shadow[node];
+      store_row<double, C>(mult64 + static_cast<int64_t>(sh) * cs64, acc);
+      store4(centers64, sh, V4<double>{cx, cy, cz, tot});
+      float accf[C];
+      for (int j = 0; j < C; ++j) accf[j] = static_cast<float>(acc[j]);
+      store_row<float, C>(mult + static_cast<int64_t>(node) * cs, accf);
+      store4(centers4, node, V4<float>{static_cast<float>(cx), static_cast<float>(cy), static_cast<float>(cz),
+                                       static_cast<float>(tot)});
+    }
+    __syncthreads();
+  }
+}
+
+// Mixed precision L2L of the top levels: loc64 starts from the float32 M2L
+// rows of the shadowed nodes; then for the nodes at depth 1..3 (shallowest
+// first) the parent's float64 local is shifted in float64 and added to the
+// child's shadow (depth <= 2 internal nodes) or, rounded once, to its float32 row.
+template <int P>
+__global__ __launch_bounds__(kWave) void l2l_top_kernel(const int32_t* __restrict__ nodes,
+                                                        const int32_t* __restrict__ lev_off, int32_t nlev,
+                                                        const int32_t* __restrict__ top_nodes, int32_t ntop,
+                                                        const int32_t* __restrict__ parent,
+                                                        const int32_t* __restrict__ shadow,
+                                                        const float* __restrict__ centers4,
+                                                        const double* __restrict__ centers64,
+                                                        float* __restrict__ locals, int32_t cs,
+                                                        double* __restrict__ loc64, int32_t cs64) {
+  constexpr int C = ncoef(P);
+  const int t = threadIdx.x;
+  if (t < ntop) {
+    const int node = top_nodes[t];
+    float tmp[C];
+    double v[C];
+    load_row<float, C>(locals + static_cast<int64_t>(node) * cs, tmp);
+    for (int j = 0; j < C; ++j) v[j] = tmp[j];
+    store_row<double, C>(loc64 + static_cast<int64_t>(shadow[node]) * cs64, v);
+  }
+  __syncthreads();
+  for (int lev = 0; lev < nlev; ++lev) {
+    const int count = lev_off[lev + 1] - lev_off[lev];
+    if (t < count) {
+      const int node = nodes[lev_off[lev] + t];
+      const int par = parent[node];
+      const int sp = shadow[par];
+      const V4<double> cp = load4(centers64, sp);
+      const int sn = shadow[node];
+      V4<double> cn;
+      if (sn >= 0) {
+        cn = load4(centers64, sn);
+      } else {
+        const V4<float> f = load4(centers4, node);
+        cn = V4<double>{f.x, f.y, f.z, f.w};
+      }
+      double v[C];
+      load_row<double, C>(loc64 + static_cast<int64_t>(sp) * cs64, v);
+      Align<double, P> a;
+      a.init(cp.x - cn.x, cp.y - cn.y, cp.z - cn.z);
+      rotate_all<double, P, Rot::ToZLocal>(v, a);
+      zshift_l2l<double, P>(v, a.r);
+      rotate_all<double, P, Rot::FromZLocal>(v, a);
+      if (sn >= 0) {
+        double w[C];
+        load_row<double, C>(loc64 + static_cast<int64_t>(sn) * cs64, w);
+        for (int j = 0; j < C; ++j) w[j] += v[j];
+        store_row<double, C>(loc64 + static_cast<int64_t>(sn) * cs64, w);
+        float wf[C];
+        for (int j = 0; j < C; ++j) wf[j] = static_cast<float>(w[j]);
+        store_row<float, C>(locals + static_cast<int64_t>(node) * cs, wf);
+      } else {
+        float w[C];
+        load_row<float, C>(locals + static_cast<int64_t>(node) * cs, w);
+        for (int j = 0; j < C; ++j) w[j] = static_cast<float>(static_cast<double>(w[j]) + v[j]);
+        store_row<float, C>(locals + static_cast<int64_t>(node) * cs, w);
+      }
+    }
+    __syncthreads();
+  }
+}
+
 inline int blocks_for(int64_t n, int b) { return static_cast<int>((n + b - 1) / b); }
 
 }  // namespace
@@ -293,6 +432,45 @@ int32_t jfm_l2l_level(void* stream, int32_t dtype, int32_t p, const int32_t* nod
                                                                 (T*)locals, cs);
   }));
   return check_launch("jfm_l2l_level");
+}
+
+int32_t jfm_m2m_top(void* stream, int32_t p, const int32_t* nodes, const int32_t* lev_off, int32_t nlev,
+                    const int32_t* lev_off_host, const int32_t* left, const int32_t* right, const int32_t* node_range,
+                    const int32_t* shadow, void* multipoles, void* centers4, void* mult64, void* centers64) {
+  if (nlev < 0 || !lev_off_host) return fail(JFM_ERR_ARG, "jfm_m2m_top: bad level table");
+  if (nlev == 0) return JFM_OK;
+  for (int l = 0; l < nlev; ++l)
+    if (lev_off_host[l + 1] - lev_off_host[l] > kWave || lev_off_host[l + 1] < lev_off_host[l])
+      return fail(JFM_ERR_ARG, "jfm_m2m_top: a top level holds more than 64 nodes");
+  if (!nodes || !lev_off || !left || !right || !node_range || !shadow || !multipoles || !centers4 || !mult64 ||
+      !centers64)
+    return fail(JFM_ERR_ARG, "jfm_m2m_top: NULL buffer");
+  hipStream_t s = as_stream(stream);
+  JFM_FOR_ORDER(p, {
+    m2m_top_kernel<P><<<1, kWave, 0, s>>>(nodes, lev_off, nlev, left, right, node_range, shadow, (float*)multipoles,
+                                          coef_stride(P, 4), (float*)centers4, (double*)mult64, coef_stride(P, 8),
+                                          (double*)centers64);
+  });
+  return check_launch("jfm_m2m_top");
+}
+
+int32_t jfm_l2l_top(void* stream, int32_t p, const int32_t* nodes, const int32_t* lev_off, int32_t nlev,
+                    const int32_t* lev_off_host, const int32_t* top_nodes, int32_t ntop, const int32_t* parent,
+                    const int32_t* shadow, const void* centers4, const void* centers64, void* locals, void* loc64) {
+  if (nlev < 0 || ntop < 0 || ntop > kWave || !lev_off_host) return fail(JFM_ERR_ARG, "jfm_l2l_top: bad sizes");
+  if (nlev == 0 && ntop == 0) return JFM_OK;
+  for (int l = 0; l < nlev; ++l)
+    if (lev_off_host[l + 1] - lev_off_host[l] > kWave || lev_off_host[l + 1] < lev_off_host[l])
+      return fail(JFM_ERR_ARG, "jfm_l2l_top: a top level holds more than 64 nodes");
+  if (!nodes || !lev_off || !top_nodes || !parent || !shadow || !centers4 || !centers64 || !locals || !loc64)
+    return fail(JFM_ERR_ARG, "jfm_l2l_top: NULL buffer");
+  hipStream_t s = as_stream(stream);
+  JFM_FOR_ORDER(p, {
+    l2l_top_kernel<P><<<1, kWave, 0, s>>>(nodes, lev_off, nlev, top_nodes, ntop, parent, shadow,
+                                          (const float*)centers4, (const double*)centers64, (float*)locals,
+                                          coef_stride(P, 4), (double*)loc64, coef_stride(P, 8));
+  });
+  return check_launch("jfm_l2l_top");
 }
 
 int32_t jfm_l2p(void* stream, int32_t dtype, int32_t p, const void* pos4, int32_t n, int32_t leaf_size,

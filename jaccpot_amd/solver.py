@@ -89,7 +89,8 @@ class FastMultipoleMethod:
     ``"rot_scale"`` (the reference's real-basis default, solver.py:723-726).
     ``precision`` picks float32 or float64 kernels; ``None`` follows the input
     dtype.  ``compensated`` (default on) selects blocked Kahan accumulation in
-    the float32 P2P sums.
+    the float32 P2P sums.  ``mixed_precision`` (float32 only) runs the M2M and
+    L2L of the top three tree levels in float64 on shadow rows.
     """
 
     def __init__(
@@ -106,6 +107,7 @@ class FastMultipoleMethod:
         precision: Optional[str] = None,
         working_dtype: Any = None,
         compensated: bool = True,
+        mixed_precision: bool = False,
         num_threads: int = 0,
         **legacy_kwargs: Any,
     ):
@@ -144,6 +146,7 @@ class FastMultipoleMethod:
         self.softening = float(softening)
         self.working_dtype = dtype
         self.compensated = bool(compensated)
+        self.mixed_precision = bool(mixed_precision)
         self.num_threads = int(num_threads)
         self.device = _require_gpu()
         self._last_state: Optional[FMMPreparedState] = None
@@ -217,6 +220,8 @@ class FastMultipoleMethod:
             locals=torch.zeros((nodes, cs), dtype=dtype, device=dev),
             compensated=self.compensated,
         )
+        if self.mixed_precision and dtype == torch.float32:
+            state.extra["mixed"] = self._mixed_schedule(topo, p, dev)
         xd = torch.as_tensor(x_host, device=dev)
         md = torch.as_tensor(m_host, device=dev)
         self.refresh_prepared_state(state, xd, md)
@@ -241,18 +246,57 @@ class FastMultipoleMethod:
         self._upward(state, s, code)
         self._downward(state, s, code)
 
+    @staticmethod
+    def _mixed_schedule(topo: Topology, p: int, dev) -> dict:
+        """Shadow rows and level tables for the float64 top three levels."""
+        I = topo.num_internal
+        depth = topo.depth
+        top = np.array([k for k in range(I) if depth[k] <= 2], np.int32)
+        shadow = np.full(topo.num_nodes, -1, np.int32)
+        shadow[top] = np.arange(top.size, dtype=np.int32)
+        m2m_levels = [top[depth[top] == d] for d in (2, 1, 0)]
+        m2m_nodes = np.concatenate(m2m_levels) if top.size else np.zeros(0, np.int32)
+        m2m_off = np.concatenate([[0], np.cumsum([len(v) for v in m2m_levels])]).astype(np.int32)
+        l2l_levels = [np.nonzero(depth == d)[0].astype(np.int32) for d in (1, 2, 3)]
+        l2l_nodes = np.concatenate(l2l_levels)
+        l2l_off = np.concatenate([[0], np.cumsum([len(v) for v in l2l_levels])]).astype(np.int32)
+
+        def dev_i32(a):
+            return torch.as_tensor(np.ascontiguousarray(a if a.size else np.zeros(1, np.int32), np.int32)).to(dev)
+
+        cs64 = nat.coeff_stride(p, torch.float64)
+        ntop = max(int(top.size), 1)
+        return {
+            "top": dev_i32(top), "ntop": int(top.size), "shadow": dev_i32(shadow),
+            "m2m_nodes": dev_i32(m2m_nodes), "m2m_off": dev_i32(m2m_off), "m2m_off_host": m2m_off,
+            "l2l_nodes": dev_i32(l2l_nodes), "l2l_off": dev_i32(l2l_off), "l2l_off_host": l2l_off,
+            "mult64": torch.zeros((ntop, cs64), dtype=torch.float64, device=dev),
+            "centers64": torch.zeros((ntop, 4), dtype=torch.float64, device=dev),
+            "loc64": torch.zeros((ntop, cs64), dtype=torch.float64, device=dev),
+        }
+
     def _upward(self, state: FMMPreparedState, s, code) -> None:
         topo = state.topology
+        mixed = state.extra.get("mixed")
         nat.call("jfm_p2m", s, code, state.order, state.pos4.data_ptr(), topo.num_particles, topo.num_leaves,
                  topo.leaf_size, topo.num_internal, state.multipoles.data_ptr(), state.centers4.data_ptr())
         off = topo.m2m_off
         base = state.m2m_nodes.data_ptr()
+        D = topo.max_depth
         for lev in range(len(off) - 1):
             a, b = int(off[lev]), int(off[lev + 1])
+            if mixed is not None and D - lev <= 2:
+                continue  # float64 top levels below
             if b > a:
                 nat.call("jfm_m2m_level", s, code, state.order, base + 4 * a, b - a, state.left.data_ptr(),
                          state.right.data_ptr(), state.node_range.data_ptr(), state.multipoles.data_ptr(),
                          state.centers4.data_ptr())
+        if mixed is not None and mixed["ntop"]:
+            oh = mixed["m2m_off_host"]
+            nat.call("jfm_m2m_top", s, state.order, mixed["m2m_nodes"].data_ptr(), mixed["m2m_off"].data_ptr(),
+                     len(oh) - 1, nat.ptr(oh), state.left.data_ptr(), state.right.data_ptr(),
+                     state.node_range.data_ptr(), mixed["shadow"].data_ptr(), state.multipoles.data_ptr(),
+                     state.centers4.data_ptr(), mixed["mult64"].data_ptr(), mixed["centers64"].data_ptr())
 
     def _downward(self, state: FMMPreparedState, s, code) -> None:
         topo = state.topology
@@ -261,9 +305,18 @@ class FastMultipoleMethod:
             nat.call("jfm_m2l", s, code, state.order, state.m2l_targets.data_ptr(), int(topo.m2l_targets.size),
                      state.far_off.data_ptr(), state.far_src.data_ptr(), state.multipoles.data_ptr(),
                      state.centers4.data_ptr(), state.locals.data_ptr())
+        mixed = state.extra.get("mixed")
+        first_lev = 0
+        if mixed is not None and mixed["ntop"]:
+            oh = mixed["l2l_off_host"]
+            nat.call("jfm_l2l_top", s, state.order, mixed["l2l_nodes"].data_ptr(), mixed["l2l_off"].data_ptr(),
+                     len(oh) - 1, nat.ptr(oh), mixed["top"].data_ptr(), mixed["ntop"], state.parent.data_ptr(),
+                     mixed["shadow"].data_ptr(), state.centers4.data_ptr(), mixed["centers64"].data_ptr(),
+                     state.locals.data_ptr(), mixed["loc64"].data_ptr())
+            first_lev = 3  # depths 1..3 done in float64
         off = topo.l2l_off
         base = state.l2l_nodes.data_ptr()
-        for lev in range(len(off) - 1):
+        for lev in range(first_lev, len(off) - 1):
             a, b = int(off[lev]), int(off[lev + 1])
             if b > a:
                 nat.call("jfm_l2l_level", s, code, state.order, base + 4 * a, b - a, state.parent.data_ptr(),

@@ -172,3 +172,34 @@ def test_target_indices_and_input_types(require_gpu):
     at = fmm.compute_accelerations(xt, mt, leaf_size=16, max_order=4)
     assert isinstance(at, torch.Tensor) and at.device.type == "cuda"
     np.testing.assert_array_equal(at.cpu().numpy(), full)
+
+
+def test_mixed_precision_top_levels(require_gpu):
+    """float32 with float64 top-three-level M2M/L2L matches the oracle to 1e-5 and
+    its float64 shadows match the oracle's top-level expansions to float32 input rounding."""
+    from jaccpot_amd import FastMultipoleMethod
+
+    x, m = ics.plummer(8192, 11)
+    x32, m32 = x.astype(np.float32), m.astype(np.float32)
+    xr, mr = x32.astype(np.float64), m32.astype(np.float64)
+    p, leaf, theta, eps = 6, 64, 0.6, 1e-3
+    fmm = FastMultipoleMethod(precision="fp32", theta=theta, softening=eps, mixed_precision=True)
+    state = fmm.prepare_state(x32, m32, leaf_size=leaf, max_order=p)
+    acc = fmm.evaluate_prepared_state(state)
+    torch.cuda.synchronize()
+    topo = OT.build_topology(xr, mr, leaf, theta)
+    M = ofmm.upward(topo, xr[topo.perm], mr[topo.perm], p)
+    Lc = ofmm.downward(topo, M, p)
+    mixed = state.extra["mixed"]
+    top = mixed["top"].cpu().numpy()[: mixed["ntop"]]
+    C = H.sh_size(p)
+    assert mixed["ntop"] == int(np.sum(topo.depth[: topo.num_internal] <= 2))
+    m64 = mixed["mult64"].cpu().numpy()[: len(top), :C]
+    assert _rel_rows(m64, M[top]) < 2e-6
+    l64 = mixed["loc64"].cpu().numpy()[: len(top), :C]
+    nz = np.abs(Lc[top]).max(axis=1) > 0
+    if nz.any():
+        assert _rel_rows(l64[nz], Lc[top][nz]) < 2e-5
+    ref = ofmm.fmm_accelerations(xr, mr, leaf_size=leaf, p=p, theta=theta, eps=eps, topo=topo)
+    rel = np.linalg.norm(acc - ref) / np.linalg.norm(ref)
+    assert rel < 1e-5, rel
