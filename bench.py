@@ -43,16 +43,6 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def m2l_flops(p: int) -> int:
-    """Algorithmic flops per M2L pair: 4 S(p) + 2 Z(p) (F(4)=718, F(6)=1964)."""
-    S = (p + 1) * (2 * p + 1) * (2 * p + 3) // 3
-
-    def T(k):
-        return k * (k + 1) // 2
-
-    return 4 * S + 2 * (T(p + 1) + 2 * sum(T(p - 2 * m + 1) for m in range(1, p // 2 + 1)))
-
-
 def cpu_baseline_sample(gen: str, p: int, leaf: int, theta: float, eps: float, n_sample: int):
     """The float64 oracle FMM (NumPy restatement, single thread) on a bounded sample."""
     from oracle import fmm as ofmm
@@ -88,6 +78,7 @@ def main() -> int:
     from jaccpot_amd import FastMultipoleMethod, direct_sum_gravitational_acceleration
     from jaccpot_amd import _native as nat
     from jaccpot_amd import ics
+    from jaccpot_amd.perf import m2l_flops
 
     if args.calibrate_copy_mib > 0:
         nel = args.calibrate_copy_mib * (1 << 20) // 4

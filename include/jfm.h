@@ -134,6 +134,18 @@ int32_t jfm_m2l(void* stream, int32_t dtype, int32_t p, const int32_t* targets, 
                 const int32_t* far_off, const int32_t* far_src, const void* multipoles,
                 const void* centers4, void* locals);
 
+/* M2L at a per-pair expansion order (gear): every pair of the CSR (far_off,
+ * far_src) is translated at order gear <= p from the source multipole truncated
+ * to degree gear, and the result is written into (accumulate == 0, other
+ * coefficients of the row zeroed) or added to (accumulate != 0) degrees <= gear
+ * of the target local.  Calling it once per gear over per-gear lists implements
+ * the per-pair order the reference's adaptive policy tags
+ * (runtime/_adaptive_policy.py:2216-2385, far_pairs_by_gear at
+ * runtime/kernels/_l2l.py:491-493). */
+int32_t jfm_m2l_gear(void* stream, int32_t dtype, int32_t p, int32_t gear, int32_t accumulate,
+                     const int32_t* targets, int32_t num_targets, const int32_t* far_off, const int32_t* far_src,
+                     const void* multipoles, const void* centers4, void* locals);
+
 /* One L2L level: locals[c] += l2l(locals[parent[c]], c_parent - c_c) for c in nodes[count].
  * Replaces _propagate_solidfmm_locals_by_level (runtime/kernels/_l2l.py:210-409). */
 int32_t jfm_l2l_level(void* stream, int32_t dtype, int32_t p, const int32_t* nodes, int32_t count,
@@ -187,6 +199,38 @@ int32_t jfm_evaluate(void* stream, int32_t dtype, int32_t p, const void* pos4, i
                      int32_t leaf_size, int32_t num_internal, const int32_t* near_off, const int32_t* near_src,
                      const void* locals, const void* centers4, double eps2, double G,
                      const int32_t* out_index, void* acc, void* pot, int32_t flags);
+
+/* ------------------------------------------------------------------------
+ * Learned per-pair M2L order (no reference counterpart: the reference tags
+ * gears with a hand-written error model, runtime/_adaptive_policy.py:2216-2384,
+ * and then runs every pair at full order, runtime/kernels/_l2l.py:491-493).
+ * Features [K][8] of far pairs (k -> target tgt_of[k], source far_src[k]):
+ * log2(d/(r_t+r_s)), log10(m_s/m_total), log10(m_t/m_total), and
+ * log10(|M_s,n|/(m_s r_s^n)) for n = 2..6; radius [nodes] float32 bounding radii.
+ * Labels [K][3]: log10 of the r_t^n-weighted relative truncation error of the
+ * order 2/4/6 translation against order 8 (multipoles8 at order 8, float32).
+ * MLP 8 -> 64 -> 64 -> 3 (ReLU), jfm_mlp_num_params() float32 parameters
+ * W1[64][8] b1 W2[64][64] b2 W3[3][64] b3, used as bfloat16 (wbf16) with
+ * float32 accumulation.  jfm_mlp_backward: gradient of the mean squared error
+ * over the K pairs (grad_and_loss[num_params] = loss), deterministic; scratch
+ * holds jfm_mlp_backward_scratch_floats(K, per_block) floats.  jfm_mlp_adam:
+ * one Adam step on the float32 masters, refreshing wbf16.  jfm_select_gears:
+ * smallest of gears3[0..2] whose predicted log10 error <= tol_log10, else gears3[2]. */
+int32_t jfm_pair_features(void* stream, int32_t p, int32_t K, const int32_t* tgt_of, const int32_t* far_src,
+                          const void* multipoles, const void* centers4, const float* radius, double total_mass,
+                          float* features);
+int32_t jfm_pair_labels(void* stream, int32_t K, const int32_t* tgt_of, const int32_t* far_src,
+                        const void* multipoles8, const void* centers4, const float* radius, float* labels);
+int32_t jfm_mlp_num_params(void);
+int64_t jfm_mlp_backward_scratch_floats(int32_t K, int32_t per_block);
+int32_t jfm_mlp_forward(void* stream, int32_t K, const float* X, const uint16_t* wbf16, float* Y);
+int32_t jfm_mlp_backward(void* stream, int32_t K, int32_t per_block, const float* X, const float* targets,
+                         const uint16_t* wbf16, float* scratch, float* grad_and_loss);
+int32_t jfm_mlp_adam(void* stream, int32_t step, float lr, float beta1, float beta2, float eps, float* master,
+                     const float* grad, float* m1, float* m2, uint16_t* wbf16);
+int32_t jfm_mlp_to_bf16(void* stream, const float* master, uint16_t* wbf16);
+int32_t jfm_select_gears(void* stream, int32_t K, const float* predictions, float tol_log10, const int32_t* gears3,
+                         int8_t* gears);
 
 /* Float64 direct sum at targets[nt] (tree slots) over all n particles of pos4
  * (float64), self excluded; writes acc [nt][3] and pot [nt] (pot may be NULL).

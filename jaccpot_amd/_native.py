@@ -50,6 +50,8 @@ _SIGNATURES = {
                                 c_void_p, c_void_p]),
     "jfm_m2l": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
                           c_void_p]),
+    "jfm_m2l_gear": (c_int32, [c_void_p, c_int32, c_int32, c_int32, c_int32, c_void_p, c_int32, c_void_p, c_void_p,
+                               c_void_p, c_void_p, c_void_p]),
     "jfm_l2l_level": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_int32, c_void_p, c_void_p, c_void_p]),
     "jfm_m2m_top": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p,
                               c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
@@ -63,6 +65,17 @@ _SIGNATURES = {
                                c_void_p, c_void_p, c_void_p, c_double, c_double, c_void_p, c_void_p, c_void_p,
                                c_int32]),
     "jfm_direct_scratch_doubles": (c_int64, [c_int32]),
+    "jfm_pair_features": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                    c_double, c_void_p]),
+    "jfm_pair_labels": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "jfm_mlp_num_params": (c_int32, []),
+    "jfm_mlp_backward_scratch_floats": (c_int64, [c_int32, c_int32]),
+    "jfm_mlp_forward": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_void_p]),
+    "jfm_mlp_backward": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "jfm_mlp_adam": (c_int32, [c_void_p, c_int32, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                               c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "jfm_mlp_to_bf16": (c_int32, [c_void_p, c_void_p, c_void_p]),
+    "jfm_select_gears": (c_int32, [c_void_p, c_int32, c_void_p, ctypes.c_float, c_void_p, c_void_p]),
     "jfm_host_operator": (c_int32, [c_int32, c_int32, c_int32, c_void_p, c_void_p, c_void_p]),
     "jfm_direct_sum_f64": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_double, c_double, c_void_p,
                                      c_void_p, c_void_p]),
@@ -105,7 +118,8 @@ def check(rc: int, name: str) -> None:
 def call(name: str, *args) -> int:
     """Call an entry point and raise JfmError on a non-zero status."""
     rc = getattr(load(), name)(*args)
-    if name not in ("jfm_direct_scratch_doubles", "jfm_coeff_stride", "jfm_order_supported", "jfm_abi_version"):
+    if name not in ("jfm_direct_scratch_doubles", "jfm_coeff_stride", "jfm_order_supported", "jfm_abi_version",
+                    "jfm_mlp_num_params", "jfm_mlp_backward_scratch_floats"):
         check(rc, name)
     return rc
 

@@ -1,0 +1,423 @@
+// Learned per-pair M2L order: pair features, order-8 truncation-error labels,
+// and the 8 -> 64 -> 64 -> 3 MLP (ReLU, bfloat16 weights, float32
+// accumulation) with its forward, backward and Adam kernels.
+//
+// Features of a far pair (target t, source s), all dimensionless:
+//   f0 = log2(d / (r_t + r_s))     separation over the summed bounding radii
+//   f1 = log10(m_s / m_total), f2 = log10(m_t / m_total)
+//   f3..f7 = log10(|M_s,n| / (m_s r_s^n) + 1e-30), n = 2..6 (|.| = L2 over m)
+// Label of a pair for gear g: the relative size of the order-g truncation
+// error against the order-8 translation over the target sphere,
+//   e_g = |W (L8 - Lg)| / |W L8|,  W_n = r_t^n,
+// stored as log10(max(e_g, 1e-7)).
+//
+// MLP parameters (float32 masters, flat): W1[64][8] b1[64] W2[64][64] b2[64]
+// W3[3][64] b3[3] = 4931 values; the kernels read a bfloat16 copy and
+// accumulate in float32.  Backward: per workgroup the per-pair activations
+// and deltas of 32 pairs at a time are staged in LDS and every thread owns a
+// fixed set of parameters, so the per-block partial gradients and their
+// final block-ordered reduction are deterministic.
+
+#include <hip/hip_runtime.h>
+
+#include "device_math.hpp"
+#include "launch.hpp"
+#include "rows.hpp"
+
+namespace jfm {
+namespace {
+
+constexpr int kIn = 8, kHid = 64, kOut = 3;
+constexpr int oW1 = 0, oB1 = oW1 + kHid * kIn, oW2 = oB1 + kHid, oB2 = oW2 + kHid * kHid, oW3 = oB2 + kHid,
+              oB3 = oW3 + kOut * kHid, kParams = oB3 + kOut;
+static_assert(kParams == 4931, "parameter layout");
+
+constexpr int kThreads = 256;
+constexpr int kSub = 32;  // pairs staged per backward sub-batch
+
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
+
+__device__ __forceinline__ uint16_t f32_to_bf16(float f) {  // round to nearest even (finite inputs)
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return static_cast<uint16_t>(u >> 16);
+}
+
+// ---------------------------------------------------------------- features
+template <int P>
+__global__ __launch_bounds__(kThreads) void features_kernel(int32_t K, const int32_t* __restrict__ tgt_of,
+                                                            const int32_t* __restrict__ far_src,
+                                                            const float* __restrict__ mult, int32_t cs,
+                                                            const float* __restrict__ centers4,
+                                                            const float* __restrict__ radius, float inv_total,
+                                                            float* __restrict__ feat) {
+  const int k = blockIdx.x * kThreads + threadIdx.x;
+  if (k >= K) return;
+  const int t = tgt_of[k], s = far_src[k];
+  const V4<float> ct = load4(centers4, t), cs4 = load4(centers4, s);
+  const float dx = ct.x - cs4.x, dy = ct.y - cs4.y, dz = ct.z - cs4.z;
+  const float d = sqrtf(dx * dx + dy * dy + dz * dz);
+  const float rt = radius[t], rs = radius[s];
+  float f[kIn];
+  f[0] = log2f(d / fmaxf(rt + rs, 1e-30f));
+  f[1] = log10f(fmaxf(cs4.w * inv_total, 1e-30f));
+  f[2] = log10f(fmaxf(ct.w * inv_total, 1e-30f));
+  const float* row = mult + static_cast<int64_t>(s) * cs;
+  float rn = rs * rs;
+  for (int n = 2; n <= 6; ++n) {
+    float sq = 0.0f;
+    if (n <= P)
+      for (int m = -n; m <= n; ++m) {
+        const float v = row[sh_idx(n, m)];
+        sq += v * v;
+      }
+    f[n + 1] = log10f(sqrtf(sq) / fmaxf(cs4.w * rn, 1e-30f) + 1e-30f);
+    rn *= rs;
+  }
+  for (int j = 0; j < kIn; ++j) feat[static_cast<int64_t>(k) * kIn + j] = f[j];
+}
+
+// ------------------------------------------------------------------ labels
+template <int G>
+__device__ __forceinline__ void m2l_order(const float* M, const Align<float, G>& a, float* L) {
+  constexpr int C = ncoef(G);
+  float v[C];
+  for (int j = 0; j < C; ++j) v[j] = M[j];
+  rotate_all<float, G, Rot::ToZMultipole>(v, a);
+  zm2l<float, G>(v, L, a.r);
+  rotate_all<float, G, Rot::FromZLocal>(L, a);
+}
+
+__global__ __launch_bounds__(64) void labels_kernel(int32_t K, const int32_t* __restrict__ tgt_of,
+                                                    const int32_t* __restrict__ far_src,
+                                                    const float* __restrict__ mult8, int32_t cs8,
+                                                    const float* __restrict__ centers4,
+                                                    const float* __restrict__ radius, float* __restrict__ labels) {
+  const int k = blockIdx.x * 64 + threadIdx.x;
+  if (k >= K) return;
+  const int t = tgt_of[k], s = far_src[k];
+  const V4<float> ct = load4(centers4, t), cs4 = load4(centers4, s);
+  const float dx = ct.x - cs4.x, dy = ct.y - cs4.y, dz = ct.z - cs4.z;
+  float M[ncoef(8)];
+  load_row<float, ncoef(8)>(mult8 + static_cast<int64_t>(s) * cs8, M);
+  float L8[ncoef(8)];
+  {
+    Align<float, 8> a;
+    a.init(dx, dy, dz);
+    m2l_order<8>(M, a, L8);
+  }
+  const float rt = radius[t];
+  float w[9];
+  w[0] = 1.0f;
+  for (int n = 1; n <= 8; ++n) w[n] = w[n - 1] * rt;
+  float ref = 0.0f;
+  for (int n = 0; n <= 8; ++n)
+    for (int m = -n; m <= n; ++m) {
+      const float q = w[n] * L8[sh_idx(n, m)];
+      ref += q * q;
+    }
+  ref = sqrtf(fmaxf(ref, 1e-37f));
+  float out[3];
+#define JFM_LABEL(IDX, G)                                              \
+  {                                                                    \
+    float Lg[ncoef(G)];                                                \
+    Align<float, G> a;                                                 \
+    a.init(dx, dy, dz);                                                \
+    m2l_order<G>(M, a, Lg);                                            \
+    float e = 0.0f;                                                    \
+    for (int n = 0; n <= 8; ++n)                                       \
+      for (int m = -n; m <= n; ++m) {                                  \
+        const float lg = (n <= G) ? Lg[sh_idx(n, m)] : 0.0f;           \
+        const float q = w[n] * (L8[sh_idx(n, m)] - lg);                \
+        e += q * q;                                                    \
+      }                                                                \
+    out[IDX] = log10f(fmaxf(sqrtf(e) / ref, 1e-7f));                   \
+  }
+  JFM_LABEL(0, 2)
+  JFM_LABEL(1, 4)
+  JFM_LABEL(2, 6)
+#undef JFM_LABEL
+  for (int j = 0; j < 3; ++j) labels[static_cast<int64_t>(k) * 3 + j] = out[j];
+}
+
+// --------------------------------------------------------------------- MLP
+__device__ __forceinline__ void stage_weights(const uint16_t* __restrict__ wbf, float* w) {
+  for (int j = threadIdx.x; j < kParams; j += blockDim.x) w[j] = bf16_to_f32(wbf[j]);
+}
+
+__device__ __forceinline__ void mlp_forward_one(const float* w, const float* x, float* h1, float* h2, float* y) {
+#pragma unroll 4
+  for (int i = 0; i < kHid; ++i) {
+    float a = w[oB1 + i];
+#pragma unroll
+    for (int j = 0; j < kIn; ++j) a = fmaf(w[oW1 + i * kIn + j], x[j], a);
+    h1[i] = fmaxf(a, 0.0f);
+  }
+#pragma unroll 2
+  for (int i = 0; i < kHid; ++i) {
+    float a = w[oB2 + i];
+#pragma unroll
+    for (int j = 0; j < kHid; ++j) a = fmaf(w[oW2 + i * kHid + j], h1[j], a);
+    h2[i] = fmaxf(a, 0.0f);
+  }
+#pragma unroll
+  for (int i = 0; i < kOut; ++i) {
+    float a = w[oB3 + i];
+#pragma unroll
+    for (int j = 0; j < kHid; ++j) a = fmaf(w[oW3 + i * kHid + j], h2[j], a);
+    y[i] = a;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void mlp_forward_kernel(int32_t K, const float* __restrict__ X,
+                                                               const uint16_t* __restrict__ wbf,
+                                                               float* __restrict__ Y) {
+  __shared__ float w[kParams];
+  stage_weights(wbf, w);
+  __syncthreads();
+  const int k = blockIdx.x * kThreads + threadIdx.x;
+  if (k >= K) return;
+  float x[kIn], h1[kHid], h2[kHid], y[kOut];
+  for (int j = 0; j < kIn; ++j) x[j] = X[static_cast<int64_t>(k) * kIn + j];
+  mlp_forward_one(w, x, h1, h2, y);
+  for (int j = 0; j < kOut; ++j) Y[static_cast<int64_t>(k) * kOut + j] = y[j];
+}
+
+// Per-workgroup partial gradient of loss = mean_k mean_j (y_kj - t_kj)^2 over
+// pairs [blockIdx.x * per_block, ...).  partial[block][kParams + 1] (last = loss sum).
+__global__ __launch_bounds__(kThreads) void mlp_backward_kernel(int32_t K, int32_t per_block,
+                                                                const float* __restrict__ X,
+                                                                const float* __restrict__ Tgt,
+                                                                const uint16_t* __restrict__ wbf,
+                                                                float* __restrict__ partial) {
+  __shared__ float w[kParams];
+  __shared__ float s_x[kSub][kIn], s_h1[kSub][kHid], s_h2[kSub][kHid], s_d1[kSub][kHid], s_d2[kSub][kHid],
+      s_dy[kSub][kOut], s_loss[kSub];
+  stage_weights(wbf, w);
+  __syncthreads();
+  const float scale = 2.0f / (static_cast<float>(K) * kOut);
+  // each thread owns parameters j = threadIdx.x + kThreads * q
+  constexpr int kOwn = (kParams + kThreads - 1) / kThreads;
+  float g[kOwn];
+  for (int q = 0; q < kOwn; ++q) g[q] = 0.0f;
+  float loss = 0.0f;
+  const int b0 = blockIdx.x * per_block;
+  const int b1 = min(K, b0 + per_block);
+  for (int base = b0; base < b1; base += kSub) {
+    const int cnt = min(kSub, b1 - base);
+    if (threadIdx.x < kSub) {
+      const int r = threadIdx.x;
+      if (r < cnt) {
+        const int k = base + r;
+        float x[kIn], h1[kHid], h2[kHid], y[kOut];
+        for (int j = 0; j < kIn; ++j) x[j] = X[static_cast<int64_t>(k) * kIn + j];
+        mlp_forward_one(w, x, h1, h2, y);
+        float dy[kOut], l = 0.0f;
+        for (int j = 0; j < kOut; ++j) {
+          const float e = y[j] - Tgt[static_cast<int64_t>(k) * kOut + j];
+          l += e * e;
+          dy[j] = scale * e;
+        }
+        float d2[kHid], d1[kHid];
+        for (int i = 0; i < kHid; ++i) {
+          float a = 0.0f;
+          for (int j = 0; j < kOut; ++j) a = fmaf(w[oW3 + j * kHid + i], dy[j], a);
+          d2[i] = h2[i] > 0.0f ? a : 0.0f;
+        }
+        for (int i = 0; i < kHid; ++i) {
+          float a = 0.0f;
+          for (int j = 0; j < kHid; ++j) a = fmaf(w[oW2 + j * kHid + i], d2[j], a);
+          d1[i] = h1[i] > 0.0f ? a : 0.0f;
+        }
+        for (int j = 0; j < kIn; ++j) s_x[r][j] = x[j];
+        for (int i = 0; i < kHid; ++i) {
+          s_h1[r][i] = h1[i];
+          s_h2[r][i] = h2[i];
+          s_d1[r][i] = d1[i];
+          s_d2[r][i] = d2[i];
+        }
+        for (int j = 0; j < kOut; ++j) s_dy[r][j] = dy[j];
+        s_loss[r] = l;
+      } else {
+        for (int j = 0; j < kIn; ++j) s_x[r][j] = 0.0f;
+        for (int i = 0; i < kHid; ++i) s_h1[r][i] = s_h2[r][i] = s_d1[r][i] = s_d2[r][i] = 0.0f;
+        for (int j = 0; j < kOut; ++j) s_dy[r][j] = 0.0f;
+        s_loss[r] = 0.0f;
+      }
+    }
+    __syncthreads();
+    for (int q = 0; q < kOwn; ++q) {
+      const int j = threadIdx.x + kThreads * q;
+      if (j >= kParams) break;
+      float a = 0.0f;
+      if (j < oB1) {
+        const int i = j / kIn, c = j % kIn;
+        for (int r = 0; r < kSub; ++r) a = fmaf(s_d1[r][i], s_x[r][c], a);
+      } else if (j < oW2) {
+        const int i = j - oB1;
+        for (int r = 0; r < kSub; ++r) a += s_d1[r][i];
+      } else if (j < oB2) {
+        const int i = (j - oW2) / kHid, c = (j - oW2) % kHid;
+        for (int r = 0; r < kSub; ++r) a = fmaf(s_d2[r][i], s_h1[r][c], a);
+      } else if (j < oW3) {
+        const int i = j - oB2;
+        for (int r = 0; r < kSub; ++r) a += s_d2[r][i];
+      } else if (j < oB3) {
+        const int i = (j - oW3) / kHid, c = (j - oW3) % kHid;
+        for (int r = 0; r < kSub; ++r) a = fmaf(s_dy[r][i], s_h2[r][c], a);
+      } else {
+        const int i = j - oB3;
+        for (int r = 0; r < kSub; ++r) a += s_dy[r][i];
+      }
+      g[q] += a;
+    }
+    if (threadIdx.x == 0)
+      for (int r = 0; r < kSub; ++r) loss += s_loss[r];
+    __syncthreads();
+  }
+  float* out = partial + static_cast<int64_t>(blockIdx.x) * (kParams + 1);
+  for (int q = 0; q < kOwn; ++q) {
+    const int j = threadIdx.x + kThreads * q;
+    if (j < kParams) out[j] = g[q];
+  }
+  if (threadIdx.x == 0) out[kParams] = loss / (static_cast<float>(K) * kOut);
+}
+
+__global__ __launch_bounds__(kThreads) void reduce_blocks_kernel(const float* __restrict__ partial, int32_t nblk,
+                                                                 float* __restrict__ grad) {
+  const int j = blockIdx.x * kThreads + threadIdx.x;
+  if (j > kParams) return;
+  float s = 0.0f;
+  for (int b = 0; b < nblk; ++b) s += partial[static_cast<int64_t>(b) * (kParams + 1) + j];
+  grad[j] = s;
+}
+
+__global__ __launch_bounds__(kThreads) void adam_kernel(int32_t n, float* __restrict__ master,
+                                                        const float* __restrict__ grad, float* __restrict__ m1,
+                                                        float* __restrict__ m2, float lr, float beta1, float beta2,
+                                                        float eps, float bc1, float bc2, uint16_t* __restrict__ wbf) {
+  const int j = blockIdx.x * kThreads + threadIdx.x;
+  if (j >= n) return;
+  const float g = grad[j];
+  const float a = beta1 * m1[j] + (1.0f - beta1) * g;
+  const float b = beta2 * m2[j] + (1.0f - beta2) * g * g;
+  m1[j] = a;
+  m2[j] = b;
+  const float upd = lr * (a / bc1) / (sqrtf(b / bc2) + eps);
+  const float w = master[j] - upd;
+  master[j] = w;
+  wbf[j] = f32_to_bf16(w);
+}
+
+__global__ __launch_bounds__(kThreads) void to_bf16_kernel(int32_t n, const float* __restrict__ src,
+                                                           uint16_t* __restrict__ dst) {
+  const int j = blockIdx.x * kThreads + threadIdx.x;
+  if (j < n) dst[j] = f32_to_bf16(src[j]);
+}
+
+// smallest gear whose predicted log10 error is <= tol; the largest gear otherwise
+__global__ __launch_bounds__(kThreads) void select_gears_kernel(int32_t K, const float* __restrict__ Y, float tol,
+                                                                int32_t g0, int32_t g1, int32_t g2,
+                                                                int8_t* __restrict__ gears) {
+  const int k = blockIdx.x * kThreads + threadIdx.x;
+  if (k >= K) return;
+  const float* y = Y + static_cast<int64_t>(k) * kOut;
+  int g = g2;
+  if (y[1] <= tol) g = g1;
+  if (y[0] <= tol) g = g0;
+  gears[k] = static_cast<int8_t>(g);
+}
+
+inline int nblocks(int64_t n, int b) { return static_cast<int>((n + b - 1) / b); }
+
+}  // namespace
+}  // namespace jfm
+
+using namespace jfm;
+
+extern "C" {
+
+int32_t jfm_mlp_num_params(void) { return kParams; }
+
+int64_t jfm_mlp_backward_scratch_floats(int32_t K, int32_t per_block) {
+  if (K <= 0 || per_block <= 0) return 0;
+  return static_cast<int64_t>(nblocks(K, per_block)) * (kParams + 1);
+}
+
+int32_t jfm_pair_features(void* stream, int32_t p, int32_t K, const int32_t* tgt_of, const int32_t* far_src,
+                          const void* multipoles, const void* centers4, const float* radius, double total_mass,
+                          float* features) {
+  if (K < 0 || !(total_mass > 0.0)) return fail(JFM_ERR_ARG, "jfm_pair_features: bad sizes");
+  if (K == 0) return JFM_OK;
+  if (!tgt_of || !far_src || !multipoles || !centers4 || !radius || !features)
+    return fail(JFM_ERR_ARG, "jfm_pair_features: NULL buffer");
+  hipStream_t s = as_stream(stream);
+  JFM_FOR_ORDER(p, {
+    features_kernel<P><<<nblocks(K, kThreads), kThreads, 0, s>>>(K, tgt_of, far_src, (const float*)multipoles,
+                                                                 coef_stride(P, 4), (const float*)centers4, radius,
+                                                                 static_cast<float>(1.0 / total_mass), features);
+  });
+  return check_launch("jfm_pair_features");
+}
+
+int32_t jfm_pair_labels(void* stream, int32_t K, const int32_t* tgt_of, const int32_t* far_src,
+                        const void* multipoles8, const void* centers4, const float* radius, float* labels) {
+  if (K < 0) return fail(JFM_ERR_ARG, "jfm_pair_labels: bad sizes");
+  if (K == 0) return JFM_OK;
+  if (!tgt_of || !far_src || !multipoles8 || !centers4 || !radius || !labels)
+    return fail(JFM_ERR_ARG, "jfm_pair_labels: NULL buffer");
+  labels_kernel<<<nblocks(K, 64), 64, 0, as_stream(stream)>>>(K, tgt_of, far_src, (const float*)multipoles8,
+                                                               coef_stride(8, 4), (const float*)centers4, radius,
+                                                               labels);
+  return check_launch("jfm_pair_labels");
+}
+
+int32_t jfm_mlp_forward(void* stream, int32_t K, const float* X, const uint16_t* wbf16, float* Y) {
+  if (K < 0) return fail(JFM_ERR_ARG, "jfm_mlp_forward: bad sizes");
+  if (K == 0) return JFM_OK;
+  if (!X || !wbf16 || !Y) return fail(JFM_ERR_ARG, "jfm_mlp_forward: NULL buffer");
+  mlp_forward_kernel<<<nblocks(K, kThreads), kThreads, 0, as_stream(stream)>>>(K, X, wbf16, Y);
+  return check_launch("jfm_mlp_forward");
+}
+
+int32_t jfm_mlp_backward(void* stream, int32_t K, int32_t per_block, const float* X, const float* targets,
+                         const uint16_t* wbf16, float* scratch, float* grad_and_loss) {
+  if (K <= 0 || per_block <= 0) return fail(JFM_ERR_ARG, "jfm_mlp_backward: bad sizes");
+  if (!X || !targets || !wbf16 || !scratch || !grad_and_loss) return fail(JFM_ERR_ARG, "jfm_mlp_backward: NULL");
+  hipStream_t s = as_stream(stream);
+  const int nb = nblocks(K, per_block);
+  mlp_backward_kernel<<<nb, kThreads, 0, s>>>(K, per_block, X, targets, wbf16, scratch);
+  int32_t rc = check_launch("jfm_mlp_backward");
+  if (rc != JFM_OK) return rc;
+  reduce_blocks_kernel<<<nblocks(kParams + 1, kThreads), kThreads, 0, s>>>(scratch, nb, grad_and_loss);
+  return check_launch("jfm_mlp_backward");
+}
+
+int32_t jfm_mlp_adam(void* stream, int32_t step, float lr, float beta1, float beta2, float eps, float* master,
+                     const float* grad, float* m1, float* m2, uint16_t* wbf16) {
+  if (step < 1) return fail(JFM_ERR_ARG, "jfm_mlp_adam: step must be >= 1");
+  if (!master || !grad || !m1 || !m2 || !wbf16) return fail(JFM_ERR_ARG, "jfm_mlp_adam: NULL buffer");
+  const float bc1 = 1.0f - powf(beta1, static_cast<float>(step));
+  const float bc2 = 1.0f - powf(beta2, static_cast<float>(step));
+  adam_kernel<<<nblocks(kParams, kThreads), kThreads, 0, as_stream(stream)>>>(kParams, master, grad, m1, m2, lr,
+                                                                             beta1, beta2, eps, bc1, bc2, wbf16);
+  return check_launch("jfm_mlp_adam");
+}
+
+int32_t jfm_mlp_to_bf16(void* stream, const float* master, uint16_t* wbf16) {
+  if (!master || !wbf16) return fail(JFM_ERR_ARG, "jfm_mlp_to_bf16: NULL buffer");
+  to_bf16_kernel<<<nblocks(kParams, kThreads), kThreads, 0, as_stream(stream)>>>(kParams, master, wbf16);
+  return check_launch("jfm_mlp_to_bf16");
+}
+
+int32_t jfm_select_gears(void* stream, int32_t K, const float* predictions, float tol_log10, const int32_t* gears3,
+                         int8_t* gears) {
+  if (K < 0 || !gears3) return fail(JFM_ERR_ARG, "jfm_select_gears: bad arguments");
+  if (K == 0) return JFM_OK;
+  if (!predictions || !gears) return fail(JFM_ERR_ARG, "jfm_select_gears: NULL buffer");
+  select_gears_kernel<<<nblocks(K, kThreads), kThreads, 0, as_stream(stream)>>>(K, predictions, tol_log10, gears3[0],
+                                                                               gears3[1], gears3[2], gears);
+  return check_launch("jfm_select_gears");
+}
+
+}  // extern "C"

@@ -91,6 +91,11 @@ class FastMultipoleMethod:
     dtype.  ``compensated`` (default on) selects blocked Kahan accumulation in
     the float32 P2P sums.  ``mixed_precision`` (float32 only) runs the M2M and
     L2L of the top three tree levels in float64 on shadow rows.
+    ``adaptive_order=True`` with a trained ``order_model``
+    (:class:`jaccpot_amd.learned.OrderModel`) evaluates each far pair at the
+    smallest gear of ``p_gears`` whose predicted log10 truncation error is at
+    most ``gear_tolerance`` (the largest gear otherwise); ``max_order`` must be
+    the largest gear.  Gears are fixed in ``prepare_state``.
     """
 
     def __init__(
@@ -108,6 +113,8 @@ class FastMultipoleMethod:
         working_dtype: Any = None,
         compensated: bool = True,
         mixed_precision: bool = False,
+        order_model: Any = None,
+        gear_tolerance: float = -5.0,
         num_threads: int = 0,
         **legacy_kwargs: Any,
     ):
@@ -118,8 +125,14 @@ class FastMultipoleMethod:
             raise NotImplementedError("only basis='real' (the Dehnen real basis) is implemented")
         if m2l_impl not in (None, "rot_scale"):
             raise NotImplementedError(f"m2l_impl={m2l_impl!r} is not implemented (use 'rot_scale')")
-        if adaptive_order or p_gears is not None:
-            raise NotImplementedError("adaptive / per-pair M2L orders are not available in this build")
+        if adaptive_order:
+            if order_model is None:
+                raise ValueError("adaptive_order=True needs a trained order_model (jaccpot_amd.learned.OrderModel)")
+            gears = tuple(int(g) for g in (p_gears if p_gears is not None else order_model.gears))
+            if gears != tuple(order_model.gears):
+                raise ValueError(f"p_gears {gears} do not match the order model's gears {order_model.gears}")
+        elif p_gears is not None:
+            raise ValueError("p_gears is only meaningful with adaptive_order=True")
         if legacy_kwargs:
             raise TypeError(f"unsupported keyword arguments: {sorted(legacy_kwargs)}")
         if not (theta > 0.0):
@@ -147,6 +160,9 @@ class FastMultipoleMethod:
         self.working_dtype = dtype
         self.compensated = bool(compensated)
         self.mixed_precision = bool(mixed_precision)
+        self.adaptive_order = bool(adaptive_order)
+        self.order_model = order_model
+        self.gear_tolerance = float(gear_tolerance)
         self.num_threads = int(num_threads)
         self.device = _require_gpu()
         self._last_state: Optional[FMMPreparedState] = None
@@ -224,9 +240,33 @@ class FastMultipoleMethod:
             state.extra["mixed"] = self._mixed_schedule(topo, p, dev)
         xd = torch.as_tensor(x_host, device=dev)
         md = torch.as_tensor(m_host, device=dev)
+        if self.adaptive_order:
+            self._prepare_gears(state, xd, md)
         self.refresh_prepared_state(state, xd, md)
         self._last_state = state
         return state
+
+    def _prepare_gears(self, state: FMMPreparedState, xd: torch.Tensor, md: torch.Tensor) -> None:
+        """Tag every far pair with a gear from the order model (prepare time, like the reference's traversal)."""
+        from .learned import build_gear_schedule, pair_features
+
+        if state.dtype != torch.float32:
+            raise NotImplementedError("learned per-pair orders run on the float32 path")
+        gears = tuple(self.order_model.gears)
+        if state.order != max(gears):
+            raise ValueError(f"max_order must equal the largest gear {max(gears)}")
+        s = nat.current_stream()
+        code = nat.dtype_code(state.dtype)
+        nat.call("jfm_gather_sorted", s, nat.dtype_code(xd.dtype), code, xd.data_ptr(), md.to(xd.dtype).data_ptr(),
+                 state.perm.data_ptr(), state.topology.num_particles, state.pos4.data_ptr())
+        self._upward(state, s, code)
+        K = state.topology.num_far_pairs
+        if K == 0:
+            return
+        X = pair_features(state)
+        g = self.order_model.select_gears(X, self.gear_tolerance).cpu().numpy()
+        state.extra["gear_of_pair"] = g
+        state.extra["gears"] = build_gear_schedule(state, g, gears)
 
     def refresh_prepared_state(self, state: FMMPreparedState, positions: torch.Tensor, masses: torch.Tensor) -> None:
         """Re-run gather + upward + downward at the state's frozen topology.
@@ -301,7 +341,14 @@ class FastMultipoleMethod:
     def _downward(self, state: FMMPreparedState, s, code) -> None:
         topo = state.topology
         state.locals.zero_()
-        if topo.m2l_targets.size:
+        sched = state.extra.get("gears")
+        if sched is not None:
+            for i, g in enumerate(sched.gears):
+                if sched.num_targets[i]:
+                    nat.call("jfm_m2l_gear", s, code, state.order, g, 1, sched.targets[i].data_ptr(),
+                             sched.num_targets[i], sched.offsets[i].data_ptr(), sched.sources[i].data_ptr(),
+                             state.multipoles.data_ptr(), state.centers4.data_ptr(), state.locals.data_ptr())
+        elif topo.m2l_targets.size:
             nat.call("jfm_m2l", s, code, state.order, state.m2l_targets.data_ptr(), int(topo.m2l_targets.size),
                      state.far_off.data_ptr(), state.far_src.data_ptr(), state.multipoles.data_ptr(),
                      state.centers4.data_ptr(), state.locals.data_ptr())

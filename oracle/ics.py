@@ -41,3 +41,27 @@ def blobs(n: int, seed: int, nblobs: int = 64, sigma: float = 0.05):
     x = centres[which] + sigma * rng.standard_normal(size=(n, 3))
     m = rng.lognormal(0.0, 0.5, size=n)
     return x, m
+
+
+def galaxy(n: int, seed: int, a: float = 1.0, rmax: float = 100.0, r_d: float = 0.3, z_d: float = 0.03,
+           halo_fraction: float = 0.8):
+    """Hernquist halo (a, truncated at rmax, r = a sqrt(u)/(1 - sqrt(u)), u ~ U(0, M(rmax)),
+    M(r) = r^2/(r+a)^2) holding halo_fraction of the equal-mass particles, plus an
+    exponential disc: R ~ Gamma(2, r_d), z ~ Laplace(0, z_d), uniform azimuth."""
+    rng = np.random.default_rng(seed)
+    nh = int(round(halo_fraction * n))
+    nd = n - nh
+    umax = rmax * rmax / (rmax + a) ** 2
+    u = rng.uniform(0.0, umax, size=nh)
+    su = np.sqrt(u)
+    r = a * su / (1.0 - su)
+    cth = rng.uniform(-1.0, 1.0, size=nh)
+    ph = rng.uniform(0.0, 2.0 * np.pi, size=nh)
+    sth = np.sqrt(1.0 - cth * cth)
+    halo = np.stack([r * sth * np.cos(ph), r * sth * np.sin(ph), r * cth], axis=1)
+    R = rng.gamma(2.0, r_d, size=nd)
+    phd = rng.uniform(0.0, 2.0 * np.pi, size=nd)
+    z = rng.laplace(0.0, z_d, size=nd)
+    disc = np.stack([R * np.cos(phd), R * np.sin(phd), z], axis=1)
+    x = np.concatenate([halo, disc], axis=0)
+    return x, np.full(n, 1.0 / n)

@@ -1,0 +1,134 @@
+"""GPU: per-gear M2L, the order MLP's kernels against the NumPy oracle, and
+what the trained network gains over the fixed-order method."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import fmm as ofmm
+from oracle import harmonics as H
+from oracle import ics
+from oracle import mlp as omlp
+from oracle import topology as OT
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def test_gear_m2l_matches_oracle(require_gpu):
+    """Random per-pair gears (fp64): locals equal the oracle's truncated-multipole M2L."""
+    from jaccpot_amd import FastMultipoleMethod
+    from jaccpot_amd import _native as nat
+    from jaccpot_amd.learned import build_gear_schedule
+
+    print("[gpu] gear m2l fp64", flush=True)
+    x, m = ics.plummer(3000, 21)
+    p, leaf, theta = 6, 16, 0.6
+    fmm = FastMultipoleMethod(precision="fp64", theta=theta, softening=1e-3)
+    state = fmm.prepare_state(x, m, leaf_size=leaf, max_order=p)
+    K = state.topology.num_far_pairs
+    g = np.random.default_rng(3).choice([2, 4, 6], size=K).astype(np.int8)
+    state.extra["gears"] = build_gear_schedule(state, g, (2, 4, 6))
+    fmm._downward(state, nat.current_stream(), nat.JFM_F64)
+    torch.cuda.synchronize()
+    topo = OT.build_topology(x, m, leaf, theta)
+    M = ofmm.upward(topo, x[topo.perm], m[topo.perm], p)
+    Lc = ofmm.downward(topo, M, p, orders=g.astype(np.int64))
+    gl = state.locals.cpu().numpy()[:, : H.sh_size(p)]
+    scale = np.maximum(np.abs(Lc).max(axis=1, keepdims=True), 1e-300)
+    assert float((np.abs(gl - Lc) / scale).max()) < 1e-12
+
+
+def test_mlp_kernels_match_oracle(require_gpu):
+    from jaccpot_amd.learned import OrderModel
+
+    print("[gpu] mlp fwd/bwd", flush=True)
+    model = OrderModel(seed=5)
+    rng = np.random.default_rng(1)
+    K = 1537
+    X = rng.normal(size=(K, 8)).astype(np.float32)
+    T = rng.normal(size=(K, 3)).astype(np.float32)
+    Xd = torch.as_tensor(X, device="cuda")
+    Td = torch.as_tensor(T, device="cuda")
+    Y = model.predict(Xd, normalised=True).cpu().numpy()
+    theta = omlp.bf16_round(model.master.cpu().numpy())
+    Yr, _ = omlp.forward(theta, X)
+    assert np.abs(Y - Yr).max() <= 2e-5 * max(1.0, np.abs(Yr).max())
+    loss, grad = model.loss_and_grad(Xd, Td, per_block=200)
+    lr_, gr = omlp.loss_and_grad(theta, X, T)
+    assert abs(float(loss) - lr_) <= 1e-5 * lr_
+    g = grad.cpu().numpy()
+    assert np.linalg.norm(g - gr) <= 1e-4 * np.linalg.norm(gr)
+    # determinism: identical gradient bits on a second call
+    _, grad2 = model.loss_and_grad(Xd, Td, per_block=200)
+    assert torch.equal(grad, grad2)
+    # Adam step moves the masters against the gradient
+    before = model.master.clone()
+    model.adam(grad, lr=1e-3)
+    step = (model.master - before).cpu().numpy()
+    assert np.all(step * g <= 1e-12)
+
+
+def _train_model(seed=0):
+    from jaccpot_amd import FastMultipoleMethod
+    from jaccpot_amd.learned import OrderModel, pair_features, pair_labels
+
+    x, m = ics.galaxy(131072, 31 + seed)
+    fmm = FastMultipoleMethod(precision="fp32", theta=0.6, softening=1e-3)
+    st = fmm.prepare_state(x.astype(np.float32), m.astype(np.float32), leaf_size=32, max_order=6)
+    X = pair_features(st)
+    T = pair_labels(st)
+    model = OrderModel(seed=seed)
+    hist = model.fit(X, T, steps=300, batch=16384, lr=3e-3, log_every=50)
+    return model, hist, X, T
+
+
+def test_trained_network_gains_over_fixed_order(require_gpu):
+    """Training lowers the loss; at equal accuracy the learned gears cut M2L work."""
+    from jaccpot_amd import FastMultipoleMethod, direct_sum_gravitational_acceleration
+    from jaccpot_amd.perf import m2l_flops
+
+    print("[gpu] training order model", flush=True)
+    model, hist, X, T = _train_model()
+    print(f"[gpu]   loss {hist[0][1]:.4f} -> {hist[-1][1]:.4f}", flush=True)
+    assert hist[-1][1] < 0.25 * hist[0][1]
+    # held-out prediction quality on the training configuration
+    Y = model.predict(X)
+    resid = (Y - T).abs().median().item()
+    assert resid < 0.35, resid
+
+    x, m = ics.galaxy(65536, 77)
+    x32, m32 = x.astype(np.float32), m.astype(np.float32)
+    tgt = np.random.default_rng(12345).choice(x.shape[0], 2048, replace=False)
+    ref = direct_sum_gravitational_acceleration(x32.astype(np.float64), m32.astype(np.float64), softening=1e-3,
+                                                target_indices=tgt).cpu().numpy()
+
+    def err(acc):
+        return float(np.median(np.linalg.norm(acc[tgt] - ref, axis=1) / np.linalg.norm(ref, axis=1)))
+
+    plain = FastMultipoleMethod(precision="fp32", theta=0.6, softening=1e-3)
+    st = plain.prepare_state(x32, m32, leaf_size=32, max_order=6)
+    e_plain = err(plain.evaluate_prepared_state(st))
+    K = st.topology.num_far_pairs
+    plain_flops = m2l_flops(6) * K
+    learned = FastMultipoleMethod(precision="fp32", theta=0.6, softening=1e-3, adaptive_order=True,
+                                  p_gears=(2, 4, 6), order_model=model, gear_tolerance=-6.0)
+    stl = learned.prepare_state(x32, m32, leaf_size=32, max_order=6)
+    e_learned = err(learned.evaluate_prepared_state(stl))
+    sched = stl.extra["gears"]
+    print(f"[gpu]   plain err {e_plain:.3e}; learned err {e_learned:.3e}; gears {sched.counts}; "
+          f"M2L flops {sched.m2l_flops / plain_flops:.3f} of fixed order", flush=True)
+    assert e_learned <= 1.5 * e_plain + 2e-6
+    assert sched.m2l_flops <= 0.8 * plain_flops
+    # the network beats an order rule without it (all pairs at the middle gear) on accuracy
+    g4 = np.full(K, 4, np.int8)
+    from jaccpot_amd.learned import build_gear_schedule
+
+    st.extra["gears"] = build_gear_schedule(st, g4, (2, 4, 6))
+    from jaccpot_amd import _native as nat
+
+    plain._downward(st, nat.current_stream(), nat.JFM_F32)
+    e_g4 = err(plain.evaluate_prepared_state(st))
+    print(f"[gpu]   all pairs at gear 4: err {e_g4:.3e}", flush=True)
+    assert e_learned < e_g4
