@@ -201,6 +201,39 @@ int32_t jfm_evaluate(void* stream, int32_t dtype, int32_t p, const void* pos4, i
                      const int32_t* out_index, void* acc, void* pot, int32_t flags);
 
 /* ------------------------------------------------------------------------
+ * Adjoint (reverse-mode) kernels: exact gradients of accelerations and
+ * potentials w.r.t. positions and masses at a frozen topology, replacing the
+ * reference's custom_vjp rules for P2P (nearfield/grad.py:354-869), L2P
+ * (operators/real_p2m_l2p.py:403-491) and M2L (pallas/m2l_real_fused.py:576-829)
+ * under differentiable_accelerations (runtime/fmm_evaluate.py:733).
+ * cot4 [N][4] holds (abar_x, abar_y, abar_z, phibar) per tree slot; grad4
+ * [N][4] accumulates (xbar, mbar) per tree slot.  Rows named *_flip hold
+ * coefficients multiplied by (-1)^n; with them the tree passes of the reverse
+ * sweep reuse the forward kernels: Lbar up = jfm_m2m_level on lbar_flip,
+ * Mbar_own = jfm_m2l on lbar_flip (output is mbar_own_flip), Mbar down =
+ * jfm_l2l_level on mbar_flip.  Order of calls: jfm_p2p_adjoint (writes grad4),
+ * jfm_leaf_adjoint_a (adds L2P terms, writes leaf rows of lbar_flip), M2M
+ * levels, M2L, jfm_center_adjoint (cbar4 [nodes][4] for the nodes with far
+ * pairs; other rows must be zero), L2L levels, jfm_com_chain (chain
+ * [L][8]), jfm_leaf_adjoint_b (P2M adjoint + centre-of-mass chain, writes
+ * gpos [rows][3], gmass [rows] at out_index). */
+int32_t jfm_p2p_adjoint(void* stream, int32_t dtype, const void* pos4, const void* cot4, int32_t n,
+                        int32_t num_leaves, int32_t leaf_size, const int32_t* near_off, const int32_t* near_src,
+                        double eps2, double G, void* grad4);
+int32_t jfm_leaf_adjoint_a(void* stream, int32_t dtype, int32_t p, const void* pos4, const void* cot4, int32_t n,
+                           int32_t leaf_size, int32_t num_internal, const void* locals, const void* centers4,
+                           double G, void* grad4, void* lbar_flip);
+int32_t jfm_center_adjoint(void* stream, int32_t dtype, int32_t p, const int32_t* nodes, int32_t num_nodes,
+                           const int32_t* far_off, const int32_t* far_src, const void* multipoles,
+                           const void* locals_own, const void* lbar_flip, const void* mbar_own_flip,
+                           const void* centers4, void* cbar4);
+int32_t jfm_com_chain(void* stream, int32_t dtype, int32_t num_leaves, int32_t num_internal, const int32_t* parent,
+                      const int32_t* node_range, const void* centers4, const void* cbar4, void* chain);
+int32_t jfm_leaf_adjoint_b(void* stream, int32_t dtype, int32_t p, const void* pos4, int32_t n, int32_t leaf_size,
+                           int32_t num_internal, const void* mbar_flip, const void* centers4, const void* chain,
+                           const void* grad4, const int32_t* out_index, void* gpos, void* gmass);
+
+/* ------------------------------------------------------------------------
  * Learned per-pair M2L order (no reference counterpart: the reference tags
  * gears with a hand-written error model, runtime/_adaptive_policy.py:2216-2384,
  * and then runs every pair at full order, runtime/kernels/_l2l.py:491-493).

@@ -65,6 +65,16 @@ _SIGNATURES = {
                                c_void_p, c_void_p, c_void_p, c_double, c_double, c_void_p, c_void_p, c_void_p,
                                c_int32]),
     "jfm_direct_scratch_doubles": (c_int64, [c_int32]),
+    "jfm_p2p_adjoint": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_int32, c_int32, c_void_p,
+                                  c_void_p, c_double, c_double, c_void_p]),
+    "jfm_leaf_adjoint_a": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_int32, c_int32, c_int32,
+                                     c_void_p, c_void_p, c_double, c_void_p, c_void_p]),
+    "jfm_center_adjoint": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_int32, c_void_p, c_void_p, c_void_p,
+                                     c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "jfm_com_chain": (c_int32, [c_void_p, c_int32, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
+                                c_void_p]),
+    "jfm_leaf_adjoint_b": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_int32, c_int32, c_int32, c_void_p,
+                                     c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "jfm_pair_features": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                     c_double, c_void_p]),
     "jfm_pair_labels": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),

@@ -1,0 +1,573 @@
+// Reverse-mode (adjoint) kernels of the fixed-topology FMM: exact gradients of
+// accelerations and potentials with respect to positions and masses.
+//
+// Structure.  M2M and L2L are exact re-expansions for the degrees kept, so
+// the far field equals sum over far pairs (t <- s) of
+//   L2P( M2L( P2M(particles of s, c_s), c_t - c_s ), c_t - x_i ),  i in t,
+// and the centres matter only through the far pairs.  The reverse pass is:
+//   leaves:    x_i += G (H a_i + grad phi phib_i) (L2P, Hessian-vector product
+//              through the gradient-coefficient identities), Lbar_leaf =
+//              -G sum_i (phib_i U(d_i) + abar_i . grad U(d_i)), d_i = c - x_i;
+//   up:        Lbar_parent += l2l^T(Lbar_child) = m2m(...)   (forward M2M on
+//              parity-flipped rows);
+//   pairs:     Mbar_s = sum_t m2l^T(Lbar_t) = P m2l(P Lbar_t, c_s - c_t)
+//              (forward M2L on flipped rows, m2l is symmetric);
+//   centres:   cbar_t = Gr(L_t^own) . Lbar_t - Gr(Mbar_t^own) . M_t
+//              + sum_s d/dd [Lbar_t . m2l(M_s, d)] - sum_s d/dd [Lbar_s . m2l(M_t, d)]
+//              (this file: center_adjoint_kernel, dual numbers for d/dd);
+//   down:      Mbar_child += m2m^T(Mbar_parent) = l2l(...) (forward L2L, flipped);
+//   leaves:    x_j += m_j grad(Mbar . U)(x_j - c), m_j += Mbar . U(x_j - c)
+//              (P2M), plus the centre-of-mass chain of every ancestor.
+// P2P has its own kernel (p2p_adjoint_kernel).  Everything is deterministic.
+//
+// Identities used (packed real basis): op(w, -d) = P op(P w, d) for the three
+// translations with P = (-1)^n, m2l^T = m2l, l2l^T = m2m at the same d.
+
+#include <hip/hip_runtime.h>
+
+#include "device_math.hpp"
+#include "launch.hpp"
+#include "rows.hpp"
+
+// Orders with adjoint kernels (the dual-number centre pass is large at p > 6).
+#define JFM_FOR_ADJ_ORDER(p, ...)                                                      \
+  switch (p) {                                                                         \
+    case 1: { constexpr int P = 1; __VA_ARGS__; } break;                               \
+    case 2: { constexpr int P = 2; __VA_ARGS__; } break;                               \
+    case 3: { constexpr int P = 3; __VA_ARGS__; } break;                               \
+    case 4: { constexpr int P = 4; __VA_ARGS__; } break;                               \
+    case 5: { constexpr int P = 5; __VA_ARGS__; } break;                               \
+    case 6: { constexpr int P = 6; __VA_ARGS__; } break;                               \
+    default:                                                                           \
+      return ::jfm::fail(JFM_ERR_UNSUPPORTED, "adjoint kernels exist for orders 1..6"); \
+  }
+
+namespace jfm {
+// Forward-mode dual number (value, one directional derivative).  Kept in its
+// own namespace so its fma/sqrt overloads are found by argument-dependent
+// lookup only and never hide the scalar ones.
+namespace ad {
+template <typename T>
+struct Dual {
+  T v, d;
+  __host__ __device__ Dual() : v(0), d(0) {}
+  __host__ __device__ Dual(double x) : v(static_cast<T>(x)), d(0) {}  // NOLINT: constants lift implicitly
+  __host__ __device__ Dual(T x, T dx) : v(x), d(dx) {}
+};
+template <typename T>
+__host__ __device__ inline Dual<T> operator+(Dual<T> a, Dual<T> b) { return {a.v + b.v, a.d + b.d}; }
+template <typename T>
+__host__ __device__ inline Dual<T> operator-(Dual<T> a, Dual<T> b) { return {a.v - b.v, a.d - b.d}; }
+template <typename T>
+__host__ __device__ inline Dual<T> operator-(Dual<T> a) { return {-a.v, -a.d}; }
+template <typename T>
+__host__ __device__ inline Dual<T> operator*(Dual<T> a, Dual<T> b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+template <typename T>
+__host__ __device__ inline Dual<T> operator/(Dual<T> a, Dual<T> b) {
+  const T q = a.v / b.v;
+  return {q, (a.d - q * b.d) / b.v};
+}
+template <typename T>
+__host__ __device__ inline Dual<T>& operator+=(Dual<T>& a, Dual<T> b) { return a = a + b; }
+template <typename T>
+__host__ __device__ inline bool operator>(Dual<T> a, Dual<T> b) { return a.v > b.v; }
+template <typename T>
+__host__ __device__ inline Dual<T> fma(Dual<T> a, Dual<T> b, Dual<T> c) {
+  return {static_cast<T>(::fma(a.v, b.v, c.v)), a.d * b.v + a.v * b.d + c.d};
+}
+template <typename T>
+__host__ __device__ inline Dual<T> sqrt(Dual<T> a) {
+  const T s = ::sqrt(a.v);
+  return {s, a.d / (T(2) * s)};
+}
+}  // namespace ad
+using ad::Dual;
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kThreads = 256;
+
+// Gradient-coefficient map: sum_j F_j dU_j/d(axis) == sum_j G_j U_j with G of degree <= P-1.
+// Transpose of the degree-lowering identities of l2p_eval.
+template <typename T, int P>
+__host__ __device__ __forceinline__ void grad_coeffs(const T* F, T* Gx, T* Gy, T* Gz) {
+  constexpr int C1 = ncoef(P - 1);
+  for (int j = 0; j < C1; ++j) Gx[j] = Gy[j] = Gz[j] = T(0);
+  for (int n = 1; n <= P; ++n) {
+    // m = 0: dx U_n^0 = -U_{n-1}^1, dy U_n^0 = -U_{n-1}^{-1}, dz U_n^0 = U_{n-1}^0
+    const T f0 = F[sh_idx(n, 0)];
+    Gz[sh_idx(n - 1, 0)] += f0;
+    if (n >= 2) {
+      Gx[sh_idx(n - 1, 1)] -= f0;
+      Gy[sh_idx(n - 1, -1)] -= f0;
+    }
+    for (int m = 1; m <= n; ++m) {
+      const T fc = F[sh_idx(n, m)], fs = F[sh_idx(n, -m)];
+      // dz
+      if (m <= n - 1) {
+        Gz[sh_idx(n - 1, m)] += fc;
+        Gz[sh_idx(n - 1, -m)] += fs;
+      }
+      // dx U_n^m = (U_{n-1}^{m-1} - U_{n-1}^{m+1}) / 2 ; dx U_n^-m = (U_{n-1}^{-(m-1)} - U_{n-1}^{-(m+1)}) / 2
+      Gx[sh_idx(n - 1, m - 1)] += T(0.5) * fc;
+      if (m - 1 >= 1) Gx[sh_idx(n - 1, -(m - 1))] += T(0.5) * fs;
+      if (m + 1 <= n - 1) {
+        Gx[sh_idx(n - 1, m + 1)] -= T(0.5) * fc;
+        Gx[sh_idx(n - 1, -(m + 1))] -= T(0.5) * fs;
+      }
+      // dy U_n^m = -(U_{n-1}^{-(m+1)} + U_{n-1}^{-(m-1)}) / 2 ; dy U_n^-m = (U_{n-1}^{m+1} + U_{n-1}^{m-1}) / 2
+      if (m - 1 >= 1) Gy[sh_idx(n - 1, -(m - 1))] -= T(0.5) * fc;
+      Gy[sh_idx(n - 1, m - 1)] += T(0.5) * fs;
+      if (m + 1 <= n - 1) {
+        Gy[sh_idx(n - 1, -(m + 1))] -= T(0.5) * fc;
+        Gy[sh_idx(n - 1, m + 1)] += T(0.5) * fs;
+      }
+    }
+  }
+}
+
+// out_j = w U_j(d) + sum_a abar_a dU_j/d(axis a) (d), all degrees <= P.
+template <typename T, int P>
+__device__ __forceinline__ void weighted_harmonics(T dx, T dy, T dz, T w, T bx, T by, T bz, T* out) {
+  T re[ntri(P)], im[ntri(P)];
+  solid_harmonics<T, P>(dx, dy, dz, re, im);
+  for (int n = 0; n <= P; ++n) {
+    {
+      T v = w * re[tri(n, 0)];
+      if (n >= 1) v = fma(bz, re[tri(n - 1, 0)], v);
+      if (n >= 2) v = fma(-bx, re[tri(n - 1, 1)], fma(-by, im[tri(n - 1, 1)], v));
+      out[sh_idx(n, 0)] = v;
+    }
+    for (int m = 1; m <= n; ++m) {
+      const T ar = re[tri(n - 1, m - 1)], ai = im[tri(n - 1, m - 1)];
+      const T br = (m + 1 <= n - 1) ? re[tri(n - 1, m + 1)] : T(0);
+      const T bi = (m + 1 <= n - 1) ? im[tri(n - 1, m + 1)] : T(0);
+      const T zr = (m <= n - 1) ? re[tri(n - 1, m)] : T(0);
+      const T zi = (m <= n - 1) ? im[tri(n - 1, m)] : T(0);
+      out[sh_idx(n, m)] = w * re[tri(n, m)] + bx * T(0.5) * (ar - br) - by * T(0.5) * (ai + bi) + bz * zr;
+      out[sh_idx(n, -m)] = w * im[tri(n, m)] + bx * T(0.5) * (ai - bi) + by * T(0.5) * (ar + br) + bz * zi;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ P2P
+// xbar_i += G [ (I s3 - 3 d d^T s5)(m_i abar_j - m_j abar_i) - d s3 (m_j phib_i + m_i phib_j) ]
+// mbar_i += G [ -d s3 . abar_j - s phib_j ],  d = x_j - x_i, over self (j != i) + near leaves.
+template <typename T, int WT, bool SAFE>
+__global__ __launch_bounds__(kThreads) void p2p_adjoint_kernel(const T* __restrict__ pos4, const T* __restrict__ cot4,
+                                                               int32_t n, int32_t leaf_size,
+                                                               const int32_t* __restrict__ near_off,
+                                                               const int32_t* __restrict__ near_src, T eps2, T G,
+                                                               T* __restrict__ grad4) {
+  constexpr int NPART = kThreads / WT;
+  constexpr int TILE = sizeof(T) == 4 ? 512 : 256;
+  constexpr int LOADS = TILE / kThreads;
+  static_assert(2 * kThreads <= 2 * TILE, "reduction buffer");
+  __shared__ V4<T> tx[TILE], tc[TILE];
+  const int leaf = blockIdx.x;
+  const int i = threadIdx.x % WT, part = threadIdx.x / WT;
+  const int first = leaf * leaf_size;
+  const int cnt = min(leaf_size, n - first);
+  const int me = first + (i < cnt ? i : 0);
+  const V4<T> xi = load4(pos4, me);
+  const V4<T> ci = load4(cot4, me);
+  const int nb = near_off[leaf];
+  const int total = (1 + near_off[leaf + 1] - nb) * leaf_size;
+  T gx = 0, gy = 0, gz = 0, gm = 0;
+  for (int base = 0; base < total; base += TILE) {
+    __syncthreads();
+    for (int l = 0; l < LOADS; ++l) {
+      const int slot = base + l * kThreads + threadIdx.x;
+      V4<T> q{T(0), T(0), T(0), T(0)}, c{T(0), T(0), T(0), T(0)};
+      if (slot < total) {
+        const int qi = slot / leaf_size;
+        const int o = slot - qi * leaf_size;
+        const int sl = qi == 0 ? leaf : near_src[nb + qi - 1];
+        const int g = sl * leaf_size + o;
+        if (g < n) {
+          q = load4(pos4, g);
+          c = load4(cot4, g);
+        }
+      }
+      tx[l * kThreads + threadIdx.x] = q;
+      tc[l * kThreads + threadIdx.x] = c;
+    }
+    __syncthreads();
+    const int trips = (min(TILE, total - base) + NPART - 1) / NPART;
+    for (int k = 0; k < trips; ++k) {
+      const V4<T> s = tx[part + k * NPART];
+      const V4<T> c = tc[part + k * NPART];
+      const T dx = s.x - xi.x, dy = s.y - xi.y, dz = s.z - xi.z;
+      const T r2 = dx * dx + dy * dy + dz * dz + eps2;
+      const T sv = SAFE ? (r2 > T(0) ? T(1) / dev_sqrt(r2) : T(0)) : T(1) / dev_sqrt(r2);
+      const T s3 = sv * sv * sv;
+      const T s5 = s3 * sv * sv;
+      // u = m_i abar_j - m_j abar_i
+      const T ux = xi.w * c.x - s.w * ci.x, uy = xi.w * c.y - s.w * ci.y, uz = xi.w * c.z - s.w * ci.z;
+      const T du = dx * ux + dy * uy + dz * uz;
+      const T pw = s.w * ci.w + xi.w * c.w;
+      gx += s3 * ux - T(3) * s5 * du * dx - s3 * pw * dx;
+      gy += s3 * uy - T(3) * s5 * du * dy - s3 * pw * dy;
+      gz += s3 * uz - T(3) * s5 * du * dz - s3 * pw * dz;
+      gm += -s3 * (dx * c.x + dy * c.y + dz * c.z) - sv * c.w;
+    }
+  }
+  __syncthreads();
+  tx[threadIdx.x] = V4<T>{gx, gy, gz, gm};
+  __syncthreads();
+  if (part == 0 && i < cnt) {
+    T ax = 0, ay = 0, az = 0, am = 0;
+    for (int q = 0; q < NPART; ++q) {
+      const V4<T> v = tx[q * WT + i];
+      ax += v.x;
+      ay += v.y;
+      az += v.z;
+      am += v.w;
+    }
+    if (eps2 > T(0)) am += ci.w / dev_sqrt(eps2);  // the self term is not in the forward potential
+    store4(grad4, first + i, V4<T>{G * ax, G * ay, G * az, G * am});
+  }
+}
+
+// ------------------------------------------------------------- leaf adjoint A
+// One wave per leaf: x_i += G (H abar_i + grad phi phib_i) into grad4, and the
+// leaf's Lbar row (parity-flipped for the M2M upward pass) as the wave sum of
+// -G (phib_i U(d_i) + abar_i . grad U(d_i)).
+template <typename T, int P>
+__global__ __launch_bounds__(kWave) void leaf_adjoint_a_kernel(const T* __restrict__ pos4,
+                                                               const T* __restrict__ cot4, int32_t n,
+                                                               int32_t leaf_size, int32_t num_internal,
+                                                               const T* __restrict__ locals, int32_t cs,
+                                                               const T* __restrict__ centers4, T G,
+                                                               T* __restrict__ grad4, T* __restrict__ lbar_flip) {
+  constexpr int C = ncoef(P);
+  __shared__ T red[kWave][C + 1];
+  const int leaf = blockIdx.x, lane = threadIdx.x;
+  const int node = num_internal + leaf;
+  const int first = leaf * leaf_size;
+  const int cnt = min(leaf_size, n - first);
+  const V4<T> c = load4(centers4, node);
+  T F[C];
+  load_row<T, C>(locals + static_cast<int64_t>(node) * cs, F);
+  T Gx[ncoef(P - 1)], Gy[ncoef(P - 1)], Gz[ncoef(P - 1)];
+  grad_coeffs<T, P>(F, Gx, Gy, Gz);
+  T acc[C];
+  for (int j = 0; j < C; ++j) acc[j] = T(0);
+  for (int i = lane; i < cnt; i += kWave) {
+    const V4<T> q = load4(pos4, first + i);
+    const V4<T> cb = load4(cot4, first + i);
+    const T dx = c.x - q.x, dy = c.y - q.y, dz = c.z - q.z;
+    T phi, g1x, g1y, g1z;
+    l2p_eval<T, P>([&F](int j) { return F[j]; }, dx, dy, dz, phi, g1x, g1y, g1z);
+    // H abar = grad of (sum_a abar_a G^a) . U
+    T V[ncoef(P - 1)];
+    for (int j = 0; j < ncoef(P - 1); ++j) V[j] = cb.x * Gx[j] + cb.y * Gy[j] + cb.z * Gz[j];
+    T phv, hx, hy, hz;
+    l2p_eval<T, P - 1>([&V](int j) { return V[j]; }, dx, dy, dz, phv, hx, hy, hz);
+    V4<T> g = load4(grad4, first + i);
+    g.x += G * (hx + g1x * cb.w);
+    g.y += G * (hy + g1y * cb.w);
+    g.z += G * (hz + g1z * cb.w);
+    store4(grad4, first + i, g);
+    T w[C];
+    weighted_harmonics<T, P>(dx, dy, dz, cb.w, cb.x, cb.y, cb.z, w);
+    for (int j = 0; j < C; ++j) acc[j] -= G * w[j];
+  }
+  for (int j = 0; j < C; ++j) red[lane][j] = acc[j];
+  __syncthreads();
+  for (int j = lane; j < cs; j += kWave) {
+    T s = T(0);
+    if (j < C) {
+      for (int l = 0; l < kWave; ++l) s += red[l][j];
+      int deg = 0;
+      while ((deg + 1) * (deg + 1) <= j) ++deg;
+      if (deg & 1) s = -s;
+    }
+    lbar_flip[static_cast<int64_t>(node) * cs + j] = s;
+  }
+}
+
+// --------------------------------------------------------- centre cotangents
+// d/dd [ A . m2l(B, d) ] along `axis` by a dual-number pass of the forward cascade.
+template <typename T, int P>
+__device__ __forceinline__ T m2l_bilinear_ddelta(const T* A, const T* B, T dx, T dy, T dz, int axis) {
+  constexpr int C = ncoef(P);
+  using D = Dual<T>;
+  D v[C], w[C];
+  for (int j = 0; j < C; ++j) v[j] = D(B[j], T(0));
+  Align<D, P> a;
+  a.init(D(dx, axis == 0 ? T(1) : T(0)), D(dy, axis == 1 ? T(1) : T(0)), D(dz, axis == 2 ? T(1) : T(0)));
+  rotate_all<D, P, Rot::ToZMultipole>(v, a);
+  zm2l<D, P>(v, w, a.r);
+  rotate_all<D, P, Rot::FromZLocal>(w, a);
+  T s = T(0);
+  for (int j = 0; j < C; ++j) s = fma(A[j], w[j].d, s);
+  return s;
+}
+
+// One wave per node u with far partners: lanes over partners v, summing
+//   + d/dd [Lbar_u . m2l(M_v, d)] at d = c_u - c_v      (pair u <- v)
+//   - d/dd [Lbar_v . m2l(M_u, d)] at d = c_v - c_u      (pair v <- u)
+// and lane 0 adds Gr(Lown_u) . Lbar_u - Gr(Mbar_own_u) . M_u.  Writes cbar4[u] (xyz).
+template <typename T, int P>
+__global__ __launch_bounds__(kWave) void center_adjoint_kernel(
+    const int32_t* __restrict__ nodes, const int32_t* __restrict__ far_off, const int32_t* __restrict__ far_src,
+    const T* __restrict__ mult, const T* __restrict__ lown, const T* __restrict__ lbar_flip,
+    const T* __restrict__ mbar_own_flip, int32_t cs, const T* __restrict__ centers4, T* __restrict__ cbar4) {
+  constexpr int C = ncoef(P);
+  __shared__ T red[kWave][4];
+  const int u = nodes[blockIdx.x];
+  const int lane = threadIdx.x;
+  const V4<T> cu = load4(centers4, u);
+  T Lu[C], Mu[C];
+  load_row<T, C>(lbar_flip + static_cast<int64_t>(u) * cs, Lu);
+  load_row<T, C>(mult + static_cast<int64_t>(u) * cs, Mu);
+  for (int j = 0; j < C; ++j) {
+    int deg = 0;
+    while ((deg + 1) * (deg + 1) <= j) ++deg;
+    if (deg & 1) Lu[j] = -Lu[j];
+  }
+  T gx = 0, gy = 0, gz = 0;
+  const int b = far_off[u], e = far_off[u + 1];
+  for (int k = b + lane; k < e; k += kWave) {
+    const int v = far_src[k];
+    const V4<T> cv = load4(centers4, v);
+    T Mv[C], Lv[C];
+    load_row<T, C>(mult + static_cast<int64_t>(v) * cs, Mv);
+    load_row<T, C>(lbar_flip + static_cast<int64_t>(v) * cs, Lv);
+    for (int j = 0; j < C; ++j) {
+      int deg = 0;
+      while ((deg + 1) * (deg + 1) <= j) ++deg;
+      if (deg & 1) Lv[j] = -Lv[j];
+    }
+    const T dx = cu.x - cv.x, dy = cu.y - cv.y, dz = cu.z - cv.z;
+    gx += m2l_bilinear_ddelta<T, P>(Lu, Mv, dx, dy, dz, 0) - m2l_bilinear_ddelta<T, P>(Lv, Mu, -dx, -dy, -dz, 0);
+    gy += m2l_bilinear_ddelta<T, P>(Lu, Mv, dx, dy, dz, 1) - m2l_bilinear_ddelta<T, P>(Lv, Mu, -dx, -dy, -dz, 1);
+    gz += m2l_bilinear_ddelta<T, P>(Lu, Mv, dx, dy, dz, 2) - m2l_bilinear_ddelta<T, P>(Lv, Mu, -dx, -dy, -dz, 2);
+  }
+  red[lane][0] = gx;
+  red[lane][1] = gy;
+  red[lane][2] = gz;
+  __syncthreads();
+  if (lane == 0) {
+    T sx = 0, sy = 0, sz = 0;
+    for (int l = 0; l < kWave; ++l) {
+      sx += red[l][0];
+      sy += red[l][1];
+      sz += red[l][2];
+    }
+    // displacement terms of L2P (target side) and P2M (source side)
+    T Lo[C], Mb[C];
+    load_row<T, C>(lown + static_cast<int64_t>(u) * cs, Lo);
+    load_row<T, C>(mbar_own_flip + static_cast<int64_t>(u) * cs, Mb);
+    for (int j = 0; j < C; ++j) {
+      int deg = 0;
+      while ((deg + 1) * (deg + 1) <= j) ++deg;
+      if (deg & 1) Mb[j] = -Mb[j];
+    }
+    T Ax[ncoef(P - 1)], Ay[ncoef(P - 1)], Az[ncoef(P - 1)];
+    grad_coeffs<T, P>(Lo, Ax, Ay, Az);
+    for (int j = 0; j < ncoef(P - 1); ++j) {
+      sx = fma(Ax[j], Lu[j], sx);
+      sy = fma(Ay[j], Lu[j], sy);
+      sz = fma(Az[j], Lu[j], sz);
+    }
+    grad_coeffs<T, P>(Mb, Ax, Ay, Az);
+    for (int j = 0; j < ncoef(P - 1); ++j) {
+      sx = fma(-Ax[j], Mu[j], sx);
+      sy = fma(-Ay[j], Mu[j], sy);
+      sz = fma(-Az[j], Mu[j], sz);
+    }
+    store4(cbar4, u, V4<T>{sx, sy, sz, T(0)});
+  }
+}
+
+// ------------------------------------------------- centre-of-mass chain per leaf
+// For every ancestor a of the leaf (and the leaf): c_a = sum m_i x_i / m_a
+// (count-weighted mean when m_a == 0).  Accumulates
+//   um = sum cbar_a / m_a,  uc = sum cbar_a / n_a (massless),  vv = sum c_a . cbar_a / m_a.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void com_chain_kernel(int32_t num_leaves, int32_t num_internal,
+                                                             const int32_t* __restrict__ parent,
+                                                             const int32_t* __restrict__ node_range,
+                                                             const T* __restrict__ centers4,
+                                                             const T* __restrict__ cbar4, T* __restrict__ chain) {
+  const int l = blockIdx.x * kThreads + threadIdx.x;
+  if (l >= num_leaves) return;
+  T umx = 0, umy = 0, umz = 0, ucx = 0, ucy = 0, ucz = 0, vv = 0;
+  for (int a = num_internal + l; a >= 0; a = parent[a]) {
+    const V4<T> c = load4(centers4, a);
+    const V4<T> g = load4(cbar4, a);
+    if (c.w > T(0)) {
+      const T inv = T(1) / c.w;
+      umx += g.x * inv;
+      umy += g.y * inv;
+      umz += g.z * inv;
+      vv += (c.x * g.x + c.y * g.y + c.z * g.z) * inv;
+    } else {
+      const T inv = T(1) / static_cast<T>(node_range[2 * a + 1] - node_range[2 * a] + 1);
+      ucx += g.x * inv;
+      ucy += g.y * inv;
+      ucz += g.z * inv;
+    }
+  }
+  store4(chain, 2 * l, V4<T>{umx, umy, umz, vv});
+  store4(chain, 2 * l + 1, V4<T>{ucx, ucy, ucz, T(0)});
+}
+
+// ------------------------------------------------------------- leaf adjoint B
+// Per particle: P2M adjoint with the pushed-down Mbar (flipped rows), the COM
+// chain, then scatter (xbar, mbar) to input order.
+template <typename T, int P>
+__global__ __launch_bounds__(kThreads) void leaf_adjoint_b_kernel(const T* __restrict__ pos4, int32_t n,
+                                                                  int32_t leaf_size, int32_t num_internal,
+                                                                  const T* __restrict__ mbar_flip, int32_t cs,
+                                                                  const T* __restrict__ centers4,
+                                                                  const T* __restrict__ chain,
+                                                                  const T* __restrict__ grad4,
+                                                                  const int32_t* __restrict__ out_index,
+                                                                  T* __restrict__ gpos, T* __restrict__ gmass) {
+  constexpr int C = ncoef(P);
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const int l = i / leaf_size;
+  const int node = num_internal + l;
+  const V4<T> q = load4(pos4, i);
+  const V4<T> c = load4(centers4, node);
+  T Mb[C];
+  load_row<T, C>(mbar_flip + static_cast<int64_t>(node) * cs, Mb);
+  for (int j = 0; j < C; ++j) {
+    int deg = 0;
+    while ((deg + 1) * (deg + 1) <= j) ++deg;
+    if (deg & 1) Mb[j] = -Mb[j];
+  }
+  T phi, gx, gy, gz;
+  l2p_eval<T, P>([&Mb](int j) { return Mb[j]; }, q.x - c.x, q.y - c.y, q.z - c.z, phi, gx, gy, gz);
+  const V4<T> um = load4(chain, 2 * l);
+  const V4<T> uc = load4(chain, 2 * l + 1);
+  const V4<T> g = load4(grad4, i);
+  const T xb = g.x + q.w * gx + q.w * um.x + uc.x;
+  const T yb = g.y + q.w * gy + q.w * um.y + uc.y;
+  const T zb = g.z + q.w * gz + q.w * um.z + uc.z;
+  const T mb = g.w + phi + q.x * um.x + q.y * um.y + q.z * um.z - um.w;
+  const int64_t row = out_index ? out_index[i] : i;
+  gpos[3 * row + 0] = xb;
+  gpos[3 * row + 1] = yb;
+  gpos[3 * row + 2] = zb;
+  gmass[row] = mb;
+}
+
+template <typename T>
+int32_t launch_p2p_adj(hipStream_t s, const T* pos4, const T* cot4, int32_t n, int32_t leaf_size,
+                       int32_t num_leaves, const int32_t* near_off, const int32_t* near_src, T eps2, T G, T* grad4) {
+  const bool safe = !(eps2 > T(0));
+#define JFM_ADJ(WT)                                                                                            \
+  {                                                                                                            \
+    if (safe)                                                                                                  \
+      p2p_adjoint_kernel<T, WT, true><<<num_leaves, kThreads, 0, s>>>(pos4, cot4, n, leaf_size, near_off,      \
+                                                                      near_src, eps2, G, grad4);               \
+    else                                                                                                       \
+      p2p_adjoint_kernel<T, WT, false><<<num_leaves, kThreads, 0, s>>>(pos4, cot4, n, leaf_size, near_off,     \
+                                                                       near_src, eps2, G, grad4);              \
+    return check_launch("jfm_p2p_adjoint");                                                                    \
+  }
+  if (leaf_size <= 16) JFM_ADJ(16)
+  if (leaf_size <= 32) JFM_ADJ(32)
+  if (leaf_size <= 64) JFM_ADJ(64)
+  if (leaf_size <= 128) JFM_ADJ(128)
+  JFM_ADJ(256)
+#undef JFM_ADJ
+}
+
+inline int nblk(int64_t n, int b) { return static_cast<int>((n + b - 1) / b); }
+
+}  // namespace
+}  // namespace jfm
+
+using namespace jfm;
+
+extern "C" {
+
+int32_t jfm_p2p_adjoint(void* stream, int32_t dtype, const void* pos4, const void* cot4, int32_t n,
+                        int32_t num_leaves, int32_t leaf_size, const int32_t* near_off, const int32_t* near_src,
+                        double eps2, double G, void* grad4) {
+  if (n < 1 || leaf_size < 1 || leaf_size > 256 || num_leaves != (n + leaf_size - 1) / leaf_size)
+    return fail(JFM_ERR_ARG, "jfm_p2p_adjoint: inconsistent sizes");
+  if (!pos4 || !cot4 || !near_off || !near_src || !grad4) return fail(JFM_ERR_ARG, "jfm_p2p_adjoint: NULL buffer");
+  hipStream_t s = as_stream(stream);
+  if (dtype == JFM_F32)
+    return launch_p2p_adj<float>(s, (const float*)pos4, (const float*)cot4, n, leaf_size, num_leaves, near_off,
+                                 near_src, static_cast<float>(eps2), static_cast<float>(G), (float*)grad4);
+  if (dtype == JFM_F64)
+    return launch_p2p_adj<double>(s, (const double*)pos4, (const double*)cot4, n, leaf_size, num_leaves, near_off,
+                                  near_src, eps2, G, (double*)grad4);
+  return fail(JFM_ERR_ARG, "jfm_p2p_adjoint: bad dtype");
+}
+
+int32_t jfm_leaf_adjoint_a(void* stream, int32_t dtype, int32_t p, const void* pos4, const void* cot4, int32_t n,
+                           int32_t leaf_size, int32_t num_internal, const void* locals, const void* centers4,
+                           double G, void* grad4, void* lbar_flip) {
+  if (n < 1 || leaf_size < 1) return fail(JFM_ERR_ARG, "jfm_leaf_adjoint_a: bad sizes");
+  if (!pos4 || !cot4 || !locals || !centers4 || !grad4 || !lbar_flip)
+    return fail(JFM_ERR_ARG, "jfm_leaf_adjoint_a: NULL buffer");
+  const int L = (n + leaf_size - 1) / leaf_size;
+  if (num_internal != L - 1) return fail(JFM_ERR_ARG, "jfm_leaf_adjoint_a: num_internal must be L-1");
+  hipStream_t s = as_stream(stream);
+  JFM_FOR_DTYPE(dtype, JFM_FOR_ADJ_ORDER(p, {
+    leaf_adjoint_a_kernel<T, P><<<L, kWave, 0, s>>>((const T*)pos4, (const T*)cot4, n, leaf_size, num_internal,
+                                                    (const T*)locals, coef_stride(P, sizeof(T)),
+                                                    (const T*)centers4, static_cast<T>(G), (T*)grad4,
+                                                    (T*)lbar_flip);
+  }));
+  return check_launch("jfm_leaf_adjoint_a");
+}
+
+int32_t jfm_center_adjoint(void* stream, int32_t dtype, int32_t p, const int32_t* nodes, int32_t num_nodes,
+                           const int32_t* far_off, const int32_t* far_src, const void* multipoles,
+                           const void* locals_own, const void* lbar_flip, const void* mbar_own_flip,
+                           const void* centers4, void* cbar4) {
+  if (num_nodes < 0) return fail(JFM_ERR_ARG, "jfm_center_adjoint: bad sizes");
+  if (num_nodes == 0) return JFM_OK;
+  if (!nodes || !far_off || !far_src || !multipoles || !locals_own || !lbar_flip || !mbar_own_flip || !centers4 ||
+      !cbar4)
+    return fail(JFM_ERR_ARG, "jfm_center_adjoint: NULL buffer");
+  hipStream_t s = as_stream(stream);
+  JFM_FOR_DTYPE(dtype, JFM_FOR_ADJ_ORDER(p, {
+    center_adjoint_kernel<T, P><<<num_nodes, kWave, 0, s>>>(nodes, far_off, far_src, (const T*)multipoles,
+                                                            (const T*)locals_own, (const T*)lbar_flip,
+                                                            (const T*)mbar_own_flip, coef_stride(P, sizeof(T)),
+                                                            (const T*)centers4, (T*)cbar4);
+  }));
+  return check_launch("jfm_center_adjoint");
+}
+
+int32_t jfm_com_chain(void* stream, int32_t dtype, int32_t num_leaves, int32_t num_internal, const int32_t* parent,
+                      const int32_t* node_range, const void* centers4, const void* cbar4, void* chain) {
+  if (num_leaves < 1 || num_internal != num_leaves - 1) return fail(JFM_ERR_ARG, "jfm_com_chain: bad sizes");
+  if (!parent || !node_range || !centers4 || !cbar4 || !chain) return fail(JFM_ERR_ARG, "jfm_com_chain: NULL");
+  hipStream_t s = as_stream(stream);
+  JFM_FOR_DTYPE(dtype, {
+    com_chain_kernel<T><<<nblk(num_leaves, kThreads), kThreads, 0, s>>>(num_leaves, num_internal, parent,
+                                                                        node_range, (const T*)centers4,
+                                                                        (const T*)cbar4, (T*)chain);
+  });
+  return check_launch("jfm_com_chain");
+}
+
+int32_t jfm_leaf_adjoint_b(void* stream, int32_t dtype, int32_t p, const void* pos4, int32_t n, int32_t leaf_size,
+                           int32_t num_internal, const void* mbar_flip, const void* centers4, const void* chain,
+                           const void* grad4, const int32_t* out_index, void* gpos, void* gmass) {
+  if (n < 1 || leaf_size < 1) return fail(JFM_ERR_ARG, "jfm_leaf_adjoint_b: bad sizes");
+  if (!pos4 || !mbar_flip || !centers4 || !chain || !grad4 || !gpos || !gmass)
+    return fail(JFM_ERR_ARG, "jfm_leaf_adjoint_b: NULL buffer");
+  hipStream_t s = as_stream(stream);
+  JFM_FOR_DTYPE(dtype, JFM_FOR_ADJ_ORDER(p, {
+    leaf_adjoint_b_kernel<T, P><<<nblk(n, kThreads), kThreads, 0, s>>>(
+        (const T*)pos4, n, leaf_size, num_internal, (const T*)mbar_flip, coef_stride(P, sizeof(T)),
+        (const T*)centers4, (const T*)chain, (const T*)grad4, out_index, (T*)gpos, (T*)gmass);
+  }));
+  return check_launch("jfm_leaf_adjoint_b");
+}
+
+}  // extern "C"

@@ -418,6 +418,14 @@ class FastMultipoleMethod:
         out_dtype = state.input_dtype if state.input_dtype in (torch.float32, torch.float64) else t.dtype
         return t.to(device=state.input_device, dtype=out_dtype)
 
+    def differentiable_accelerations(self, state: FMMPreparedState, positions, masses, *,
+                                     return_potential: bool = False):
+        """Accelerations at the state's frozen topology, differentiable w.r.t. positions and masses
+        (solver.py:1358; torch.autograd with libjfm's adjoint kernels)."""
+        from .autodiff import differentiable_accelerations
+
+        return differentiable_accelerations(self, state, positions, masses, return_potential=return_potential)
+
     def compute_accelerations(self, positions, masses, *, target_indices=None, bounds=None, leaf_size: int = 16,
                               max_order: int = 4, return_potential: bool = False, theta: Optional[float] = None,
                               reuse_prepared_state: bool = False, max_acc_derivative_order: int = 0):

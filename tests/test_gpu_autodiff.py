@@ -1,0 +1,128 @@
+"""GPU: exact FMM gradients (adjoint kernels) against central finite differences
+at the frozen topology, and against the direct-sum gradient."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import ics
+from oracle.direct import direct_sum_vjp
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def _loss_and_grad(fmm, state, x, m, w, u):
+    from jaccpot_amd.autodiff import fmm_vjp
+
+    xd = torch.as_tensor(x, device="cuda").to(state.dtype)
+    md = torch.as_tensor(m, device="cuda").to(state.dtype)
+    wd = torch.as_tensor(w, device="cuda").to(state.dtype)
+    ud = torch.as_tensor(u, device="cuda").to(state.dtype)
+    gx, gm = fmm_vjp(fmm, state, xd, md, wd, ud)
+    return gx.cpu().numpy().astype(np.float64), gm.cpu().numpy().astype(np.float64)
+
+
+def _loss(fmm, state, x, m, w, u):
+    xd = torch.as_tensor(x, device="cuda").to(state.dtype)
+    md = torch.as_tensor(m, device="cuda").to(state.dtype)
+    fmm.refresh_prepared_state(state, xd, md)
+    n = x.shape[0]
+    acc = torch.empty((n, 3), dtype=state.dtype, device="cuda")
+    pot = torch.empty((n,), dtype=state.dtype, device="cuda")
+    fmm._evaluate_device(state, acc, pot)
+    return float((acc.double().cpu().numpy() * w).sum() + (pot.double().cpu().numpy() * u).sum())
+
+
+CASES = [
+    # name, gen, n, leaf, p, theta, eps
+    ("near_only_leaf64", "uniform", 300, 64, 4, 0.6, 1e-2),
+    ("plummer1200_p4", "plummer", 1200, 16, 4, 0.6, 1e-2),
+    ("uniform2000_p3_eps0", "uniform", 2000, 8, 3, 0.5, 0.0),
+    ("blobs1500_p6", "blobs", 1500, 16, 6, 0.7, 1e-2),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_fp64_gradient_matches_central_differences(case, require_gpu):
+    from jaccpot_amd import FastMultipoleMethod
+
+    name, gen, n, leaf, p, theta, eps = case
+    print(f"[gpu] adjoint fd {name}", flush=True)
+    x, m = getattr(ics, gen)(n, 41)
+    rng = np.random.default_rng(7)
+    w = rng.normal(size=(n, 3))
+    u = rng.normal(size=n)
+    fmm = FastMultipoleMethod(precision="fp64", theta=theta, softening=eps)
+    state = fmm.prepare_state(x, m, leaf_size=leaf, max_order=p)
+    gx, gm = _loss_and_grad(fmm, state, x, m, w, u)
+    scale = np.linalg.norm(gx) + np.linalg.norm(gm)
+    for k in range(6):
+        vx = rng.normal(size=x.shape)
+        vm = rng.normal(size=m.shape) * np.abs(m).mean()
+        h = 1e-6
+        fd = (_loss(fmm, state, x + h * vx, m + h * vm, w, u) - _loss(fmm, state, x - h * vx, m - h * vm, w, u)) / (
+            2 * h)
+        an = float((gx * vx).sum() + (gm * vm).sum())
+        print(f"[gpu]   dir {k}: fd {fd:.10e} adjoint {an:.10e} rel {abs(fd - an) / max(abs(an), 1e-30):.2e}",
+              flush=True)
+        assert abs(fd - an) <= 1e-6 * max(abs(an), 1e-3 * scale)
+
+
+def test_gradient_close_to_direct_sum_gradient(require_gpu):
+    """At N=4096 the FMM gradient approaches the direct-sum gradient (truncation level)."""
+    from jaccpot_amd import FastMultipoleMethod
+
+    x, m = ics.blobs(4096, 4)
+    rng = np.random.default_rng(8)
+    tgt = rng.choice(4096, 1024, replace=False)
+    eps = 1e-2
+    fmm = FastMultipoleMethod(precision="fp64", theta=0.5, softening=eps)
+    state = fmm.prepare_state(x, m, leaf_size=32, max_order=6)
+    acc, pot = fmm.evaluate_prepared_state(state, return_potential=True)
+    # C5-style loss over 1024 targets: sum_t m_t phi_t + 0.5 sum_t |a_t|^2
+    w = np.zeros((4096, 3))
+    u = np.zeros(4096)
+    w[tgt] = acc[tgt]
+    u[tgt] = m[tgt]
+    gx, gm = _loss_and_grad(fmm, state, x, m, w, u)
+    # the phi term's explicit mass dependence: d/dm_t (m_t phi_t) = phi_t
+    gm[tgt] += pot[tgt]
+    from oracle.direct import direct_sum
+
+    ad, pd = direct_sum(x, m, eps=eps, return_potential=True)
+    wd = np.zeros((4096, 3))
+    ud = np.zeros(4096)
+    wd[tgt] = ad[tgt]
+    ud[tgt] = m[tgt]
+    rx, rm = direct_sum_vjp(x, m, wd, ud, eps=eps)
+    rm[tgt] += pd[tgt]
+    ex = np.linalg.norm(gx - rx) / np.linalg.norm(rx)
+    em = np.linalg.norm(gm - rm) / np.linalg.norm(rm)
+    print(f"[gpu] grad vs direct-sum grad: positions {ex:.3e} masses {em:.3e}", flush=True)
+    assert ex < 1e-3 and em < 1e-3
+
+
+def test_autograd_function_and_fp32(require_gpu):
+    from jaccpot_amd import FastMultipoleMethod
+
+    x, m = ics.plummer(3000, 9)
+    fmm32 = FastMultipoleMethod(precision="fp32", theta=0.6, softening=1e-2)
+    st32 = fmm32.prepare_state(x.astype(np.float32), m.astype(np.float32), leaf_size=16, max_order=4)
+    xt = torch.as_tensor(x, dtype=torch.float32, device="cuda").requires_grad_(True)
+    mt = torch.as_tensor(m, dtype=torch.float32, device="cuda").requires_grad_(True)
+    acc, pot = fmm32.differentiable_accelerations(st32, xt, mt, return_potential=True)
+    loss = (mt * pot).sum() + 0.5 * (acc * acc).sum()
+    loss.backward()
+    fmm64 = FastMultipoleMethod(precision="fp64", theta=0.6, softening=1e-2)
+    st64 = fmm64.prepare_state(x, m, leaf_size=16, max_order=4)
+    x64 = torch.as_tensor(x, device="cuda").requires_grad_(True)
+    m64 = torch.as_tensor(m, device="cuda").requires_grad_(True)
+    a64, p64 = fmm64.differentiable_accelerations(st64, x64, m64, return_potential=True)
+    l64 = (m64 * p64).sum() + 0.5 * (a64 * a64).sum()
+    l64.backward()
+    ex = (torch.linalg.norm(xt.grad.double() - x64.grad) / torch.linalg.norm(x64.grad)).item()
+    em = (torch.linalg.norm(mt.grad.double() - m64.grad) / torch.linalg.norm(m64.grad)).item()
+    print(f"[gpu] fp32 vs fp64 gradient: positions {ex:.3e} masses {em:.3e}", flush=True)
+    assert ex < 1e-4 and em < 1e-4

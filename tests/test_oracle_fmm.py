@@ -85,3 +85,27 @@ def test_per_pair_orders_reduce_to_fixed_order():
     lo = H.m2l(H.truncate(M[s], 6, 2), d, 2)
     hi = H.m2l(M[s], d, 6)
     assert np.abs(lo[0] - hi[0]) < 0.05 * abs(hi[0])
+
+
+def test_direct_sum_vjp_matches_finite_differences():
+    from oracle.direct import direct_sum_vjp
+
+    rng = np.random.default_rng(4)
+    x = rng.normal(size=(40, 3))
+    m = rng.uniform(0.5, 1.5, 40)
+    ab = rng.normal(size=(40, 3))
+    pb = rng.normal(size=40)
+    for eps in (0.0, 0.05):
+        gx, gm = direct_sum_vjp(x, m, ab, pb, G=1.3, eps=eps)
+
+        def loss(xx, mm):
+            a, phi = direct_sum(xx, mm, G=1.3, eps=eps, return_potential=True)
+            return float((a * ab).sum() + (phi * pb).sum())
+
+        for _ in range(5):
+            vx = rng.normal(size=x.shape)
+            vm = rng.normal(size=m.shape)
+            h = 1e-6
+            fd = (loss(x + h * vx, m + h * vm) - loss(x - h * vx, m - h * vm)) / (2 * h)
+            an = float((gx * vx).sum() + (gm * vm).sum())
+            assert abs(fd - an) <= 1e-6 * max(1.0, abs(an))
