@@ -248,7 +248,10 @@ int32_t jfm_leaf_adjoint_b(void* stream, int32_t dtype, int32_t p, const void* p
  * over the K pairs (grad_and_loss[num_params] = loss), deterministic; scratch
  * holds jfm_mlp_backward_scratch_floats(K, per_block) floats.  jfm_mlp_adam:
  * one Adam step on the float32 masters, refreshing wbf16.  jfm_select_gears:
- * smallest of gears3[0..2] whose predicted log10 error <= tol_log10, else gears3[2]. */
+ * per far pair of the CSR (far_off [num_nodes+1], far_src), with predictions
+ * [K,3] in CSR order and w = log10(m_s / d^2): the smallest of gears3[0..2]
+ * with y_g + w <= budget_t + tol_log10, where budget_t is the largest
+ * y_top + w over target t's pairs; gears3[2] otherwise. */
 int32_t jfm_pair_features(void* stream, int32_t p, int32_t K, const int32_t* tgt_of, const int32_t* far_src,
                           const void* multipoles, const void* centers4, const float* radius, double total_mass,
                           float* features);
@@ -262,7 +265,8 @@ int32_t jfm_mlp_backward(void* stream, int32_t K, int32_t per_block, const float
 int32_t jfm_mlp_adam(void* stream, int32_t step, float lr, float beta1, float beta2, float eps, float* master,
                      const float* grad, float* m1, float* m2, uint16_t* wbf16);
 int32_t jfm_mlp_to_bf16(void* stream, const float* master, uint16_t* wbf16);
-int32_t jfm_select_gears(void* stream, int32_t K, const float* predictions, float tol_log10, const int32_t* gears3,
+int32_t jfm_select_gears(void* stream, int32_t num_nodes, const int32_t* far_off, const int32_t* far_src,
+                         const float* predictions, const void* centers4, float tol_log10, const int32_t* gears3,
                          int8_t* gears);
 
 /* Float64 direct sum at targets[nt] (tree slots) over all n particles of pos4

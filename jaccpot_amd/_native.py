@@ -85,7 +85,8 @@ _SIGNATURES = {
     "jfm_mlp_adam": (c_int32, [c_void_p, c_int32, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "jfm_mlp_to_bf16": (c_int32, [c_void_p, c_void_p, c_void_p]),
-    "jfm_select_gears": (c_int32, [c_void_p, c_int32, c_void_p, ctypes.c_float, c_void_p, c_void_p]),
+    "jfm_select_gears": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_float,
+                                   c_void_p, c_void_p]),
     "jfm_host_operator": (c_int32, [c_int32, c_int32, c_int32, c_void_p, c_void_p, c_void_p]),
     "jfm_direct_sum_f64": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_double, c_double, c_void_p,
                                      c_void_p, c_void_p]),

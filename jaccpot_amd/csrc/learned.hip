@@ -315,17 +315,39 @@ __global__ __launch_bounds__(kThreads) void to_bf16_kernel(int32_t n, const floa
   if (j < n) dst[j] = f32_to_bf16(src[j]);
 }
 
-// smallest gear whose predicted log10 error is <= tol; the largest gear otherwise
-__global__ __launch_bounds__(kThreads) void select_gears_kernel(int32_t K, const float* __restrict__ Y, float tol,
+// Per target node t (one thread), over its far pairs k in [far_off[t], far_off[t+1]):
+// w_k = log10(m_s / d^2) scales a pair's predicted relative error y_k,g to an
+// absolute force-error proxy.  The budget of t is the largest proxy of the
+// top gear over its pairs (the fixed-order method's dominant error term);
+// pair k takes the smallest gear with y_k,g + w_k <= budget + tol, the top
+// gear otherwise.
+__global__ __launch_bounds__(kThreads) void select_gears_kernel(int32_t num_nodes, const int32_t* __restrict__ far_off,
+                                                                const int32_t* __restrict__ far_src,
+                                                                const float* __restrict__ Y,
+                                                                const float* __restrict__ centers4, float tol,
                                                                 int32_t g0, int32_t g1, int32_t g2,
                                                                 int8_t* __restrict__ gears) {
-  const int k = blockIdx.x * kThreads + threadIdx.x;
-  if (k >= K) return;
-  const float* y = Y + static_cast<int64_t>(k) * kOut;
-  int g = g2;
-  if (y[1] <= tol) g = g1;
-  if (y[0] <= tol) g = g0;
-  gears[k] = static_cast<int8_t>(g);
+  const int t = blockIdx.x * kThreads + threadIdx.x;
+  if (t >= num_nodes) return;
+  const int lo = far_off[t], hi = far_off[t + 1];
+  if (lo == hi) return;
+  const V4<float> ct = load4(centers4, t);
+  auto weight = [&](int k) {
+    const V4<float> cs4 = load4(centers4, far_src[k]);
+    const float dx = ct.x - cs4.x, dy = ct.y - cs4.y, dz = ct.z - cs4.z;
+    const float d2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-37f);
+    return log10f(fmaxf(cs4.w, 1e-37f) / d2);
+  };
+  float budget = -INFINITY;
+  for (int k = lo; k < hi; ++k) budget = fmaxf(budget, Y[static_cast<int64_t>(k) * kOut + 2] + weight(k));
+  for (int k = lo; k < hi; ++k) {
+    const float thr = budget + tol - weight(k);
+    const float* y = Y + static_cast<int64_t>(k) * kOut;
+    int g = g2;
+    if (y[1] <= thr) g = g1;
+    if (y[0] <= thr) g = g0;
+    gears[k] = static_cast<int8_t>(g);
+  }
 }
 
 inline int nblocks(int64_t n, int b) { return static_cast<int>((n + b - 1) / b); }
@@ -410,13 +432,16 @@ int32_t jfm_mlp_to_bf16(void* stream, const float* master, uint16_t* wbf16) {
   return check_launch("jfm_mlp_to_bf16");
 }
 
-int32_t jfm_select_gears(void* stream, int32_t K, const float* predictions, float tol_log10, const int32_t* gears3,
+int32_t jfm_select_gears(void* stream, int32_t num_nodes, const int32_t* far_off, const int32_t* far_src,
+                         const float* predictions, const void* centers4, float tol_log10, const int32_t* gears3,
                          int8_t* gears) {
-  if (K < 0 || !gears3) return fail(JFM_ERR_ARG, "jfm_select_gears: bad arguments");
-  if (K == 0) return JFM_OK;
-  if (!predictions || !gears) return fail(JFM_ERR_ARG, "jfm_select_gears: NULL buffer");
-  select_gears_kernel<<<nblocks(K, kThreads), kThreads, 0, as_stream(stream)>>>(K, predictions, tol_log10, gears3[0],
-                                                                               gears3[1], gears3[2], gears);
+  if (num_nodes < 0 || !gears3) return fail(JFM_ERR_ARG, "jfm_select_gears: bad arguments");
+  if (num_nodes == 0) return JFM_OK;
+  if (!far_off || !far_src || !predictions || !centers4 || !gears)
+    return fail(JFM_ERR_ARG, "jfm_select_gears: NULL buffer");
+  select_gears_kernel<<<nblocks(num_nodes, kThreads), kThreads, 0, as_stream(stream)>>>(
+      num_nodes, far_off, far_src, predictions, (const float*)centers4, tol_log10, gears3[0], gears3[1], gears3[2],
+      gears);
   return check_launch("jfm_select_gears");
 }
 

@@ -93,9 +93,11 @@ class FastMultipoleMethod:
     L2L of the top three tree levels in float64 on shadow rows.
     ``adaptive_order=True`` with a trained ``order_model``
     (:class:`jaccpot_amd.learned.OrderModel`) evaluates each far pair at the
-    smallest gear of ``p_gears`` whose predicted log10 truncation error is at
-    most ``gear_tolerance`` (the largest gear otherwise); ``max_order`` must be
-    the largest gear.  Gears are fixed in ``prepare_state``.
+    smallest gear of ``p_gears`` whose predicted force error (relative
+    truncation error times m_s / d^2) is at most ``10**gear_tolerance`` times
+    the largest such error of the top gear among the target's pairs (the
+    largest gear otherwise); ``max_order`` must be the largest gear.  Gears
+    are fixed in ``prepare_state``.
     """
 
     def __init__(
@@ -114,7 +116,7 @@ class FastMultipoleMethod:
         compensated: bool = True,
         mixed_precision: bool = False,
         order_model: Any = None,
-        gear_tolerance: float = -5.0,
+        gear_tolerance: float = -1.0,
         num_threads: int = 0,
         **legacy_kwargs: Any,
     ):
@@ -264,7 +266,7 @@ class FastMultipoleMethod:
         if K == 0:
             return
         X = pair_features(state)
-        g = self.order_model.select_gears(X, self.gear_tolerance).cpu().numpy()
+        g = self.order_model.select_gears(X, self.gear_tolerance, state).cpu().numpy()
         state.extra["gear_of_pair"] = g
         state.extra["gears"] = build_gear_schedule(state, g, gears)
 
