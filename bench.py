@@ -39,6 +39,16 @@ WORKLOADS = {
 }
 
 
+# Learned+mixed configuration per workload: the per-pair order model (gears
+# 2/4/6) chooses M2L orders, the top three levels' M2M/L2L run in float64;
+# leaf size and opening angle are chosen for the lowest time at median force
+# error <= 1e-4 against the FP64 direct sum.
+LEARNED = {
+    "plummer4m_p6": {"leaf": 32, "theta": 0.75, "gears": (2, 4, 6), "gear_tolerance": -1.0},
+    "plummer1m_p4": {"leaf": 32, "theta": 0.75, "gears": (2, 4, 6), "gear_tolerance": -1.0},
+}
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -61,6 +71,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="plummer4m_p6", choices=sorted(WORKLOADS))
+    ap.add_argument("--mode", default="learned", choices=("learned", "plain"),
+                    help="learned: time the learned+mixed FMM (value) and the plain one beside it")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=16384)
     ap.add_argument("--error-targets", type=int, default=4096)
@@ -93,66 +105,116 @@ def main() -> int:
     gen, N, seed, p, leaf, theta, eps, precision = WORKLOADS[args.workload]
     log(f"workload {args.workload}: {gen} N={N} p={p} leaf={leaf} theta={theta} eps={eps} {precision}")
     x, m = getattr(ics, gen)(N, seed)
-    fmm = FastMultipoleMethod(precision=precision, theta=theta, softening=eps, compensated=True)
-    t0 = time.perf_counter()
-    state = fmm.prepare_state(x, m, leaf_size=leaf, max_order=p)
-    torch.cuda.synchronize()
-    prep_s = time.perf_counter() - t0
-    topo = state.topology
-    log(f"prepared in {prep_s:.2f}s (host topology {topo.build_seconds:.2f}s): L={topo.num_leaves} "
-        f"nodes={topo.num_nodes} depth={topo.max_depth} far pairs={topo.num_far_pairs} near={topo.num_near_entries}")
+    learned_cfg = LEARNED.get(args.workload) if args.mode == "learned" else None
 
-    dev = state.device
-    wdt = state.dtype
-    xd = torch.as_tensor(x, device=dev).to(wdt)
-    md = torch.as_tensor(m, device=dev).to(wdt)
-    acc = torch.empty((N, 3), dtype=wdt, device=dev)
-    code = nat.dtype_code(wdt)
-    flags = nat.EVAL_COMPENSATED
+    def measure(fmm, leaf_size, order, tag):
+        """Prepare once, then W warmup + K timed steps with per-stage HIP events."""
+        t0 = time.perf_counter()
+        state = fmm.prepare_state(x, m, leaf_size=leaf_size, max_order=order)
+        torch.cuda.synchronize()
+        prep_s = time.perf_counter() - t0
+        topo = state.topology
+        log(f"[{tag}] prepared in {prep_s:.2f}s (host topology {topo.build_seconds:.2f}s): L={topo.num_leaves} "
+            f"nodes={topo.num_nodes} depth={topo.max_depth} far pairs={topo.num_far_pairs} "
+            f"near={topo.num_near_entries}")
+        dev = state.device
+        wdt = state.dtype
+        xd = torch.as_tensor(x, device=dev).to(wdt)
+        md = torch.as_tensor(m, device=dev).to(wdt)
+        acc = torch.empty((N, 3), dtype=wdt, device=dev)
+        code = nat.dtype_code(wdt)
+        flags = nat.EVAL_COMPENSATED
 
-    def run_step(ev=None):
-        s = nat.current_stream()
-        if ev is not None:
-            ev[0].record()
-        nat.call("jfm_gather_sorted", s, code, code, xd.data_ptr(), md.data_ptr(), state.perm.data_ptr(), N,
-                 state.pos4.data_ptr())
-        fmm._upward(state, s, code)
-        if ev is not None:
-            ev[1].record()
-        fmm._downward(state, s, code)
-        if ev is not None:
-            ev[2].record()
-        nat.call("jfm_l2p", s, code, p, state.pos4.data_ptr(), N, leaf, topo.num_internal, state.locals.data_ptr(),
-                 state.centers4.data_ptr(), state.G, state.perm.data_ptr(), acc.data_ptr(), None)
-        if ev is not None:
-            ev[3].record()
-        nat.call("jfm_p2p", s, code, state.pos4.data_ptr(), N, topo.num_leaves, leaf, state.near_off.data_ptr(),
-                 state.near_src.data_ptr(), eps * eps, state.G, state.perm.data_ptr(), acc.data_ptr(), None, flags, 1)
-        if ev is not None:
-            ev[4].record()
+        def run_step(ev=None):
+            s = nat.current_stream()
+            if ev is not None:
+                ev[0].record()
+            nat.call("jfm_gather_sorted", s, code, code, xd.data_ptr(), md.data_ptr(), state.perm.data_ptr(), N,
+                     state.pos4.data_ptr())
+            fmm._upward(state, s, code)
+            if ev is not None:
+                ev[1].record()
+            fmm._downward(state, s, code)
+            if ev is not None:
+                ev[2].record()
+            nat.call("jfm_l2p", s, code, order, state.pos4.data_ptr(), N, leaf_size, topo.num_internal,
+                     state.locals.data_ptr(), state.centers4.data_ptr(), state.G, state.perm.data_ptr(),
+                     acc.data_ptr(), None)
+            if ev is not None:
+                ev[3].record()
+            nat.call("jfm_p2p", s, code, state.pos4.data_ptr(), N, topo.num_leaves, leaf_size,
+                     state.near_off.data_ptr(), state.near_src.data_ptr(), eps * eps, state.G,
+                     state.perm.data_ptr(), acc.data_ptr(), None, flags, 1)
+            if ev is not None:
+                ev[4].record()
 
-    for _ in range(args.warmup):
-        run_step()
-    torch.cuda.synchronize()
-    events = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    for k in range(args.steps):
-        run_step(events[k])
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t_start
-    ms_per_step = 1e3 * elapsed / args.steps
-    stage = {"upward": [], "downward": [], "l2p": [], "p2p": []}
-    for ev in events:
-        stage["upward"].append(ev[0].elapsed_time(ev[1]))
-        stage["downward"].append(ev[1].elapsed_time(ev[2]))
-        stage["l2p"].append(ev[2].elapsed_time(ev[3]))
-        stage["p2p"].append(ev[3].elapsed_time(ev[4]))
-    stage_ms = {k: float(np.mean(v)) for k, v in stage.items()}
-    log(f"{ms_per_step:.3f} ms/step; stages (ms) {json.dumps({k: round(v, 3) for k, v in stage_ms.items()})}")
+        for _ in range(args.warmup):
+            run_step()
+        torch.cuda.synchronize()
+        events = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        t_start = time.perf_counter()
+        for k in range(args.steps):
+            run_step(events[k])
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t_start
+        ms_per_step = 1e3 * elapsed / args.steps
+        stage = {"upward": [], "downward": [], "l2p": [], "p2p": []}
+        for ev in events:
+            stage["upward"].append(ev[0].elapsed_time(ev[1]))
+            stage["downward"].append(ev[1].elapsed_time(ev[2]))
+            stage["l2p"].append(ev[2].elapsed_time(ev[3]))
+            stage["p2p"].append(ev[3].elapsed_time(ev[4]))
+        stage_ms = {k: float(np.mean(v)) for k, v in stage.items()}
+        log(f"[{tag}] {ms_per_step:.3f} ms/step; stages (ms) "
+            f"{json.dumps({k: round(v, 3) for k, v in stage_ms.items()})}")
+        err = None
+        if args.error_targets > 0:
+            rng = np.random.default_rng(12345)
+            tgt = np.sort(rng.choice(N, size=min(args.error_targets, N), replace=False))
+            a_full = torch.empty((N, 3), dtype=wdt, device=dev)
+            fmm._evaluate_device(state, a_full, None)
+            if "ref" not in ref_cache:  # the same targets for every mode
+                ref_cache["ref"] = direct_sum_gravitational_acceleration(x, m, softening=eps, target_indices=tgt)
+            ref = ref_cache["ref"]
+            got = a_full[torch.as_tensor(tgt, device=dev)].to(torch.float64)
+            rel = (torch.linalg.norm(got - ref, dim=1) / torch.linalg.norm(ref, dim=1)).cpu().numpy()
+            err = {"median": float(np.median(rel)), "p99": float(np.percentile(rel, 99)), "max": float(rel.max()),
+                   "targets": int(tgt.size), "target_seed": 12345}
+            log(f"[{tag}] force error vs FP64 direct sum: median {err['median']:.3e} p99 {err['p99']:.3e}")
+        sched = state.extra.get("gears")
+        far_flops = (sched.m2l_flops if sched is not None else m2l_flops(order) * topo.num_far_pairs)
+        info = {"ms_per_step": ms_per_step, "particles_per_s": N / (ms_per_step * 1e-3), "stage_ms": stage_ms,
+                "force_error": err, "prepare_s": prep_s, "topology_build_s": topo.build_seconds,
+                "leaf_size": leaf_size, "far_pairs": topo.num_far_pairs, "m2l_gflop_per_step": far_flops / 1e9,
+                "p2p_interactions": int(topo.p2p_interactions())}
+        if sched is not None:
+            info["gear_counts"] = {str(k): v for k, v in sched.counts.items()}
+        del state, xd, md, acc
+        torch.cuda.empty_cache()
+        return info
+
+    ref_cache = {}
+    plain = measure(FastMultipoleMethod(precision=precision, theta=theta, softening=eps, compensated=True), leaf, p,
+                    "plain")
+    learned = None
+    if learned_cfg is not None:
+        from jaccpot_amd.train_order import DEFAULT_WEIGHTS, load
+
+        if not DEFAULT_WEIGHTS.exists():
+            raise SystemExit(f"order-model weights {DEFAULT_WEIGHTS} missing: run python -m jaccpot_amd.train_order")
+        model = load(DEFAULT_WEIGHTS)
+        lfmm = FastMultipoleMethod(precision=precision, theta=learned_cfg["theta"], softening=eps, compensated=True,
+                                   mixed_precision=True, adaptive_order=True, p_gears=tuple(model.gears),
+                                   order_model=model, gear_tolerance=learned_cfg["gear_tolerance"])
+        learned = measure(lfmm, learned_cfg["leaf"], max(model.gears), "learned")
+    head = learned if learned is not None else plain
+    ms_per_step = head["ms_per_step"]
+    stage_ms = head["stage_ms"]
+    topo_leaf = head["leaf_size"]
 
     # ---- roofline of the dominant kernel (P2P, FP32 VALU) -------------------
-    interactions = topo.p2p_interactions()
+    interactions = head["p2p_interactions"]
     p2p_flops = 20.0 * interactions
     p2p_s = stage_ms["p2p"] * 1e-3
     achieved = p2p_flops / p2p_s / 1e12
@@ -167,20 +229,6 @@ def main() -> int:
                 "traffic": traffic, "interactions_per_launch": int(interactions), "flops_per_interaction": 20,
                 "avg_launch_ms": round(stage_ms["p2p"], 4)}
 
-    # ---- force error against the FP64 direct sum ------------------------------
-    err = None
-    if args.error_targets > 0:
-        rng = np.random.default_rng(12345)
-        tgt = np.sort(rng.choice(N, size=min(args.error_targets, N), replace=False))
-        a_full = torch.empty((N, 3), dtype=wdt, device=dev)
-        fmm._evaluate_device(state, a_full, None)
-        ref = direct_sum_gravitational_acceleration(x, m, softening=eps, target_indices=tgt)
-        got = a_full[torch.as_tensor(tgt, device=dev)].to(torch.float64)
-        rel = (torch.linalg.norm(got - ref, dim=1) / torch.linalg.norm(ref, dim=1)).cpu().numpy()
-        err = {"median": float(np.median(rel)), "p99": float(np.percentile(rel, 99)), "max": float(rel.max()),
-               "targets": int(tgt.size), "target_seed": 12345}
-        log(f"force error vs FP64 direct sum: median {err['median']:.3e} p99 {err['p99']:.3e}")
-
     cpu = None
     if not args.no_cpu_baseline:
         cn = min(args.cpu_sample, N)
@@ -192,6 +240,16 @@ def main() -> int:
         log(f"cpu baseline {rate:.3e} particles/s on N={cn} ({dt:.1f}s)")
 
     value = N / (ms_per_step * 1e-3)
+    if learned is not None:
+        mode = (f"learned per-pair M2L order (gears {learned_cfg['gears']}, tol {learned_cfg['gear_tolerance']}) + "
+                f"float64 top-3-level M2M/L2L, leaf={learned_cfg['leaf']} theta={learned_cfg['theta']}, "
+                f"compensated P2P")
+        workload = (f"{gen} N={N} float32, learned+mixed FMM (leaf={learned_cfg['leaf']}, "
+                    f"theta={learned_cfg['theta']}, gears {learned_cfg['gears']}) vs plain fixed-order p={p} "
+                    f"leaf={leaf} theta={theta}")
+    else:
+        mode = "plain fixed-order, compensated P2P"
+        workload = f"{gen} N={N} p={p} leaf={leaf} theta={theta} softening={eps} {precision}, fixed-order plain FMM"
     out = {
         "metric": METRIC,
         "value": value,
@@ -205,18 +263,15 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32" if precision == "fp32" else "f64",
         "data": "synthetic",
-        "config": {"workload": f"{gen} N={N} p={p} leaf={leaf} theta={theta} softening={eps} {precision}, "
-                               f"fixed-order plain FMM", "N": N, "seed": seed, "order": p, "leaf_size": leaf,
-                   "theta": theta, "softening": eps, "parallelism": "single-gpu",
-                   "mode": "plain fixed-order, compensated P2P"},
+        "config": {"workload": workload, "N": N, "seed": seed, "order": p, "leaf_size": topo_leaf,
+                   "softening": eps, "parallelism": "single-gpu", "mode": mode},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "force_error": err,
+        "force_error": head["force_error"],
         "stage_ms": stage_ms,
-        "prepare_s": prep_s,
-        "topology_build_s": topo.build_seconds,
-        "far_pairs": topo.num_far_pairs,
-        "m2l_gflop_per_step": m2l_flops(p) * topo.num_far_pairs / 1e9,
+        "plain": plain,
+        "learned_mixed": learned,
+        "speedup_vs_plain": (learned["particles_per_s"] / plain["particles_per_s"]) if learned else None,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -8,6 +8,8 @@
 // leaves, each leaf_size slots wide) stream through LDS in tiles; inside a
 // tile partition q takes slots q, q + NPART, ... so every lane of a wave reads
 // one broadcast address per interaction and the tail tile stays balanced.
+// The next tile's sources are fetched into registers while the current one is
+// consumed.
 // Interactions use the hardware v_rsq_f32 (float32); per-tile partial sums are
 // folded into the running total with Kahan compensation when requested; the
 // partitions are reduced through LDS in fixed order (deterministic).
@@ -46,6 +48,50 @@ __device__ __forceinline__ void kahan_add(T& s, T& c, T v) {
   s = t;
 }
 
+// Sources of slots first + l * STRIDE (l < LOADS) of a target leaf's slot
+// sequence (self leaf, then its nnear near leaves, leaf_size slots each) into
+// registers; valid[l] is false for slots past `total` or past the last
+// particle (the caller stores those as zero-mass sources).  The loads are
+// unconditional: indices are clamped into range (an empty near list reads
+// `fallback`, a valid int32 location, and discards the value), so all
+// near-list loads are in flight together, then all position loads, which stay
+// in flight until the caller stores them.  For leaf_size == WT (power-of-two
+// leaves) the slot decode is a shift and a mask.
+template <typename T, int WT, int LOADS, int STRIDE>
+__device__ __forceinline__ void fetch_sources(const T* __restrict__ pos4, int n, int leaf_size, int leaf,
+                                              const int32_t* __restrict__ near_list, int nnear,
+                                              const int32_t* __restrict__ fallback, int total, int first,
+                                              V4<T> (&out)[LOADS], bool (&valid)[LOADS]) {
+  int qi[LOADS], o[LOADS];
+  if (leaf_size == WT) {
+#pragma unroll
+    for (int l = 0; l < LOADS; ++l) {
+      const unsigned slot = static_cast<unsigned>(first + l * STRIDE);
+      qi[l] = static_cast<int>(slot / WT);
+      o[l] = static_cast<int>(slot % WT);
+    }
+  } else {
+#pragma unroll
+    for (int l = 0; l < LOADS; ++l) {
+      const int slot = first + l * STRIDE;
+      qi[l] = slot / leaf_size;
+      o[l] = slot - qi[l] * leaf_size;
+    }
+  }
+  const int32_t* list = nnear > 0 ? near_list : fallback;
+  const int hi = max(nnear - 1, 0);
+  int v[LOADS];
+#pragma unroll
+  for (int l = 0; l < LOADS; ++l) v[l] = list[min(max(qi[l] - 1, 0), hi)];
+#pragma unroll
+  for (int l = 0; l < LOADS; ++l) {
+    const int sl = qi[l] > 0 ? v[l] : leaf;
+    const int g = first + l * STRIDE < total ? sl * leaf_size + o[l] : n;
+    valid[l] = g < n;
+    out[l] = load4(pos4, min(g, n - 1));
+  }
+}
+
 // SAFE: guard r^2 == 0 (unsoftened self term / coincident points).
 template <typename T, int WT, bool POT, bool COMP, bool SAFE>
 __global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos4, int32_t n, int32_t leaf_size,
@@ -73,24 +119,29 @@ __global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos
   const int nsl = 1 + (near_off[leaf + 1] - nb);
   const int total = nsl * leaf_size;
 
+  // The next tile's sources are fetched into registers while the current
+  // tile is being consumed, hiding the global-load latency behind the
+  // interaction loop.
+  V4<T> next[LOADS];
+  bool ok[LOADS];
+  fetch_sources<T, WT, LOADS, kThreads>(pos4, n, leaf_size, leaf, near_src + nb, nsl - 1, near_off, total,
+                                        threadIdx.x, next, ok);
+
   T sx = 0, sy = 0, sz = 0, sp = 0;
   T kx = 0, ky = 0, kz = 0, kp = 0;
   for (int base = 0; base < total; base += TILE) {
     __syncthreads();
 #pragma unroll
     for (int l = 0; l < LOADS; ++l) {
-      const int slot = base + l * kThreads + threadIdx.x;
-      V4<T> q{T(0), T(0), T(0), T(0)};
-      if (slot < total) {
-        const int qi = slot / leaf_size;
-        const int o = slot - qi * leaf_size;
-        const int sl = qi == 0 ? leaf : near_src[nb + qi - 1];
-        const int g = sl * leaf_size + o;
-        if (g < n) q = load4(pos4, g);
-      }
+      V4<T> q = next[l];
+      if (!ok[l]) q.w = T(0);
       tile[l * kThreads + threadIdx.x] = q;
     }
     __syncthreads();
+    // unconditional (past the end it yields invalid slots): a branch here
+    // would make the compiler wait for the loads at its join
+    fetch_sources<T, WT, LOADS, kThreads>(pos4, n, leaf_size, leaf, near_src + nb, nsl - 1, near_off, total,
+                                          base + TILE + threadIdx.x, next, ok);
     // Slots past `total` hold zero-mass sources, so every partition runs the
     // same (wave-uniform, scalar-controlled) trip count.
     const int trips = (min(TILE, total - base) + NPART - 1) / NPART;
@@ -129,7 +180,9 @@ __global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos
   __syncthreads();
   if (part == 0 && i < cnt) {
     T ax = 0, ay = 0, az = 0, ap = 0, cx = 0, cy = 0, cz = 0, cp = 0;
-#pragma unroll
+    // partial unroll: a full one hoists 2 * NPART row loads into registers,
+    // which for narrow leaves (NPART = 16) costs occupancy in the whole kernel
+#pragma unroll 2
     for (int q = 0; q < NPART; ++q) {
       const V4<T> v = tile[q * WT + i];
       const V4<T> c = tile[kThreads + q * WT + i];
@@ -162,163 +215,16 @@ __global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos
   }
 }
 
-// Leaf widths from which the two-targets-per-thread kernel is launched.
-#ifndef JFM_P2P_PAIRED_MIN_WT
-#define JFM_P2P_PAIRED_MIN_WT 32
-#endif
-
-// Register-blocked variant: each thread owns two targets (i and i + WT/2) of
-// the leaf, so every broadcast LDS read of a source feeds two independent
-// interaction chains.  Thread t: target pair t % (WT/2), partition t / (WT/2).
-template <typename T, int WT, bool POT, bool COMP, bool SAFE>
-__global__ __launch_bounds__(kThreads) void p2p2_kernel(const T* __restrict__ pos4, int32_t n, int32_t leaf_size,
-                                                        const int32_t* __restrict__ near_off,
-                                                        const int32_t* __restrict__ near_src, T eps2, T G,
-                                                        const int32_t* __restrict__ out_index, T* __restrict__ acc,
-                                                        T* __restrict__ pot, int32_t accumulate) {
-  constexpr int HW = WT / 2;
-  constexpr int NPART = kThreads / HW;
-  constexpr int TILE = sizeof(T) == 4 ? 1024 : 512;
-  constexpr int LOADS = TILE / kThreads;
-  static_assert(4 * kThreads * sizeof(V4<T>) <= TILE * sizeof(V4<T>) * (sizeof(T) == 4 ? 1 : 2),
-                "reduction buffer must fit");
-  __shared__ V4<T> tile[sizeof(T) == 4 ? TILE : 2 * TILE];
-
-  const int leaf = xcd_contiguous(blockIdx.x, gridDim.x);
-  const int i0 = threadIdx.x % HW;
-  const int i1 = i0 + HW;
-  const int part = threadIdx.x / HW;
-  const int first = leaf * leaf_size;
-  const int cnt = min(leaf_size, n - first);
-  const V4<T> xa = load4(pos4, first + (i0 < cnt ? i0 : 0));
-  const V4<T> xb = load4(pos4, first + (i1 < cnt ? i1 : 0));
-  const int nb = near_off[leaf];
-  const int nsl = 1 + (near_off[leaf + 1] - nb);
-  const int total = nsl * leaf_size;
-
-  T sa[4] = {0, 0, 0, 0}, ka[4] = {0, 0, 0, 0}, sb[4] = {0, 0, 0, 0}, kb[4] = {0, 0, 0, 0};
-  for (int base = 0; base < total; base += TILE) {
-    __syncthreads();
-#pragma unroll
-    for (int l = 0; l < LOADS; ++l) {
-      const int slot = base + l * kThreads + threadIdx.x;
-      V4<T> q{T(0), T(0), T(0), T(0)};
-      if (slot < total) {
-        const int qi = slot / leaf_size;
-        const int o = slot - qi * leaf_size;
-        const int sl = qi == 0 ? leaf : near_src[nb + qi - 1];
-        const int g = sl * leaf_size + o;
-        if (g < n) q = load4(pos4, g);
-      }
-      tile[l * kThreads + threadIdx.x] = q;
-    }
-    __syncthreads();
-    const int trips = (min(TILE, total - base) + NPART - 1) / NPART;
-    T ta[4] = {0, 0, 0, 0}, tb[4] = {0, 0, 0, 0};
-#pragma unroll 4
-    for (int k = 0; k < trips; ++k) {
-      const V4<T> s = tile[part + k * NPART];
-      {
-        const T dx = s.x - xa.x, dy = s.y - xa.y, dz = s.z - xa.z;
-        const T r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, eps2)));
-        T rinv = fast_rsqrt(r2);
-        if (SAFE) rinv = r2 > T(0) ? rinv : T(0);
-        const T mr = s.w * rinv;
-        const T mr3 = mr * (rinv * rinv);
-        ta[0] = fma(mr3, dx, ta[0]);
-        ta[1] = fma(mr3, dy, ta[1]);
-        ta[2] = fma(mr3, dz, ta[2]);
-        if (POT) ta[3] += mr;
-      }
-      {
-        const T dx = s.x - xb.x, dy = s.y - xb.y, dz = s.z - xb.z;
-        const T r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, eps2)));
-        T rinv = fast_rsqrt(r2);
-        if (SAFE) rinv = r2 > T(0) ? rinv : T(0);
-        const T mr = s.w * rinv;
-        const T mr3 = mr * (rinv * rinv);
-        tb[0] = fma(mr3, dx, tb[0]);
-        tb[1] = fma(mr3, dy, tb[1]);
-        tb[2] = fma(mr3, dz, tb[2]);
-        if (POT) tb[3] += mr;
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      if (c == 3 && !POT) continue;
-      if (COMP) {
-        kahan_add(sa[c], ka[c], ta[c]);
-        kahan_add(sb[c], kb[c], tb[c]);
-      } else {
-        sa[c] += ta[c];
-        sb[c] += tb[c];
-      }
-    }
-  }
-  __syncthreads();
-  // rows: [0, 256) sums of target a, [256, 512) of b, then the compensations
-  tile[threadIdx.x] = V4<T>{sa[0], sa[1], sa[2], sa[3]};
-  tile[kThreads + threadIdx.x] = V4<T>{sb[0], sb[1], sb[2], sb[3]};
-  tile[2 * kThreads + threadIdx.x] = V4<T>{ka[0], ka[1], ka[2], ka[3]};
-  tile[3 * kThreads + threadIdx.x] = V4<T>{kb[0], kb[1], kb[2], kb[3]};
-  __syncthreads();
-  if (threadIdx.x < WT) {
-    const int i = threadIdx.x;
-    if (i < cnt) {
-      const int half = i / HW;   // 0: target a, 1: target b
-      const int r = i % HW;
-      T ax = 0, ay = 0, az = 0, ap = 0, cx = 0, cy = 0, cz = 0, cp = 0;
-#pragma unroll
-      for (int q = 0; q < NPART; ++q) {
-        const V4<T> v = tile[half * kThreads + q * HW + r];
-        const V4<T> c = tile[(2 + half) * kThreads + q * HW + r];
-        ax += v.x;
-        ay += v.y;
-        az += v.z;
-        ap += v.w;
-        cx += c.x;
-        cy += c.y;
-        cz += c.z;
-        cp += c.w;
-      }
-      ax -= cx;
-      ay -= cy;
-      az -= cz;
-      ap -= cp;
-      if (POT && eps2 > T(0)) ap -= load4(pos4, first + i).w * fast_rsqrt(eps2);
-      const int64_t row = out_index ? out_index[first + i] : first + i;
-      if (accumulate) {
-        acc[3 * row + 0] += G * ax;
-        acc[3 * row + 1] += G * ay;
-        acc[3 * row + 2] += G * az;
-        if (POT) pot[row] += -G * ap;
-      } else {
-        acc[3 * row + 0] = G * ax;
-        acc[3 * row + 1] = G * ay;
-        acc[3 * row + 2] = G * az;
-        if (POT) pot[row] = -G * ap;
-      }
-    }
-  }
-}
-
 template <typename T, int WT>
 int32_t launch_p2p_wt(hipStream_t s, const T* pos4, int32_t n, int32_t leaf_size, int32_t num_leaves,
                       const int32_t* near_off, const int32_t* near_src, T eps2, T G, const int32_t* out_index,
                       T* acc, T* pot, bool comp, bool accumulate) {
   const bool want_pot = pot != nullptr;
   const bool safe = !(eps2 > T(0));
-#define JFM_P2P_LAUNCH(POT, COMP, SAFE)                                                                         \
-  do {                                                                                                        \
-    if constexpr (WT >= JFM_P2P_PAIRED_MIN_WT)                                                                \
-      p2p2_kernel<T, WT, POT, COMP, SAFE><<<num_leaves, kThreads, 0, s>>>(pos4, n, leaf_size, near_off,       \
-                                                                          near_src, eps2, G, out_index, acc,  \
-                                                                          pot, accumulate ? 1 : 0);           \
-    else                                                                                                      \
-      p2p_kernel<T, WT, POT, COMP, SAFE><<<num_leaves, kThreads, 0, s>>>(pos4, n, leaf_size, near_off,        \
-                                                                         near_src, eps2, G, out_index, acc,   \
-                                                                         pot, accumulate ? 1 : 0);            \
-  } while (0)
+#define JFM_P2P_LAUNCH(POT, COMP, SAFE)                                                                  \
+  p2p_kernel<T, WT, POT, COMP, SAFE><<<num_leaves, kThreads, 0, s>>>(pos4, n, leaf_size, near_off, near_src, \
+                                                                     eps2, G, out_index, acc, pot,           \
+                                                                     accumulate ? 1 : 0)
   if (want_pot) {
     if (comp) {
       if (safe) JFM_P2P_LAUNCH(true, true, true); else JFM_P2P_LAUNCH(true, true, false);
