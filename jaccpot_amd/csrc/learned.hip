@@ -7,9 +7,13 @@
 //   f1 = log10(m_s / m_total), f2 = log10(m_t / m_total)
 //   f3..f7 = log10(|M_s,n| / (m_s r_s^n) + 1e-30), n = 2..6 (|.| = L2 over m)
 // Label of a pair for gear g: the relative size of the order-g truncation
-// error against the order-8 translation over the target sphere,
-//   e_g = |W (L8 - Lg)| / |W L8|,  W_n = r_t^n,
-// stored as log10(max(e_g, 1e-7)).
+// error of the pair's force against the order-8 translation over the target
+// sphere,
+//   e_g = sqrt(sum_n w_n^2 N_n(L8 - Lg)^2) / sqrt(sum_n w_n^2 N_n(L8)^2),
+//   w_n = n r_t^(n-1),  N_n(L)^2 = sum_m (L_n^m)^2 / ((n+|m|)! (n-|m|)!) x (1 if m = 0 else 1/2),
+// where N_n is the rotation-invariant size of a degree-n local term in this
+// real basis and w_n the size of a degree-n gradient at radius r_t; stored as
+// log10(max(e_g, 1e-7)).
 //
 // MLP parameters (float32 masters, flat): W1[64][8] b1[64] W2[64][64] b2[64]
 // W3[3][64] b3[3] = 4931 values; the kernels read a bfloat16 copy and
@@ -78,6 +82,13 @@ __global__ __launch_bounds__(kThreads) void features_kernel(int32_t K, const int
 }
 
 // ------------------------------------------------------------------ labels
+// sqrt(c_nm), c_nm = 1 / ((n+|m|)! (n-|m|)!) x (1 if m = 0 else 1/2)
+__device__ __forceinline__ float degree_norm_root(int n, int m) {
+  const int a = m < 0 ? -m : m;
+  const double c = (a == 0 ? 1.0 : 0.5) / (kFact[n + a] * kFact[n - a]);
+  return static_cast<float>(sqrt(c));
+}
+
 template <int G>
 __device__ __forceinline__ void m2l_order(const float* M, const Align<float, G>& a, float* L) {
   constexpr int C = ncoef(G);
@@ -107,31 +118,35 @@ __global__ __launch_bounds__(64) void labels_kernel(int32_t K, const int32_t* __
     m2l_order<8>(M, a, L8);
   }
   const float rt = radius[t];
-  float w[9];
-  w[0] = 1.0f;
-  for (int n = 1; n <= 8; ++n) w[n] = w[n - 1] * rt;
+  float w[9];  // n r_t^(n-1)
+  float rp = 1.0f;
+  w[0] = 0.0f;
+  for (int n = 1; n <= 8; ++n) {
+    w[n] = static_cast<float>(n) * rp;
+    rp *= rt;
+  }
   float ref = 0.0f;
   for (int n = 0; n <= 8; ++n)
     for (int m = -n; m <= n; ++m) {
-      const float q = w[n] * L8[sh_idx(n, m)];
+      const float q = w[n] * degree_norm_root(n, m) * L8[sh_idx(n, m)];
       ref += q * q;
     }
   ref = sqrtf(fmaxf(ref, 1e-37f));
   float out[3];
-#define JFM_LABEL(IDX, G)                                              \
-  {                                                                    \
-    float Lg[ncoef(G)];                                                \
-    Align<float, G> a;                                                 \
-    a.init(dx, dy, dz);                                                \
-    m2l_order<G>(M, a, Lg);                                            \
-    float e = 0.0f;                                                    \
-    for (int n = 0; n <= 8; ++n)                                       \
-      for (int m = -n; m <= n; ++m) {                                  \
-        const float lg = (n <= G) ? Lg[sh_idx(n, m)] : 0.0f;           \
-        const float q = w[n] * (L8[sh_idx(n, m)] - lg);                \
-        e += q * q;                                                    \
-      }                                                                \
-    out[IDX] = log10f(fmaxf(sqrtf(e) / ref, 1e-7f));                   \
+#define JFM_LABEL(IDX, G)                                                             \
+  {                                                                                   \
+    float Lg[ncoef(G)];                                                               \
+    Align<float, G> a;                                                                \
+    a.init(dx, dy, dz);                                                               \
+    m2l_order<G>(M, a, Lg);                                                           \
+    float e = 0.0f;                                                                   \
+    for (int n = 0; n <= 8; ++n)                                                      \
+      for (int m = -n; m <= n; ++m) {                                                 \
+        const float lg = (n <= G) ? Lg[sh_idx(n, m)] : 0.0f;                          \
+        const float q = w[n] * degree_norm_root(n, m) * (L8[sh_idx(n, m)] - lg);      \
+        e += q * q;                                                                   \
+      }                                                                               \
+    out[IDX] = log10f(fmaxf(sqrtf(e) / ref, 1e-7f));                                  \
   }
   JFM_LABEL(0, 2)
   JFM_LABEL(1, 4)

@@ -246,6 +246,9 @@ template <typename T>
 int32_t launch_p2p(hipStream_t s, const T* pos4, int32_t n, int32_t leaf_size, int32_t num_leaves,
                    const int32_t* near_off, const int32_t* near_src, T eps2, T G, const int32_t* out_index, T* acc,
                    T* pot, bool comp, bool accumulate) {
+  if (leaf_size <= 8)
+    return launch_p2p_wt<T, 8>(s, pos4, n, leaf_size, num_leaves, near_off, near_src, eps2, G, out_index, acc,
+                               pot, comp, accumulate);
   if (leaf_size <= 16)
     return launch_p2p_wt<T, 16>(s, pos4, n, leaf_size, num_leaves, near_off, near_src, eps2, G, out_index, acc,
                                 pot, comp, accumulate);

@@ -116,7 +116,7 @@ class FastMultipoleMethod:
         compensated: bool = True,
         mixed_precision: bool = False,
         order_model: Any = None,
-        gear_tolerance: float = -1.0,
+        gear_tolerance: float = -2.5,
         num_threads: int = 0,
         **legacy_kwargs: Any,
     ):

@@ -40,6 +40,34 @@ def test_gear_m2l_matches_oracle(require_gpu):
     assert float((np.abs(gl - Lc) / scale).max()) < 1e-12
 
 
+def test_pair_labels_match_oracle(require_gpu):
+    """GPU labels (float32, order-8 upward on the device) against the float64
+    label oracle on the same topology, where the error is above float32 noise."""
+    from jaccpot_amd import FastMultipoleMethod
+    from jaccpot_amd.learned import pair_labels
+    from oracle import labels as OL
+
+    print("[gpu] pair labels", flush=True)
+    x, m = ics.plummer(3000, 23)
+    leaf, theta = 16, 0.6
+    fmm = FastMultipoleMethod(precision="fp32", theta=theta, softening=1e-3)
+    state = fmm.prepare_state(x.astype(np.float32), m.astype(np.float32), leaf_size=leaf, max_order=6)
+    got = pair_labels(state).cpu().numpy().astype(np.float64)
+    x32 = x.astype(np.float32).astype(np.float64)
+    m32 = m.astype(np.float32).astype(np.float64)
+    topo = OT.build_topology(x32, m32, leaf, theta)
+    assert np.array_equal(topo.far_src, state.topology.far_src)
+    M8 = ofmm.upward(topo, x32[topo.perm], m32[topo.perm], 8)
+    tgt = np.repeat(np.arange(topo.num_nodes), np.diff(topo.far_off))
+    src = topo.far_src
+    want = OL.pair_labels(M8[src], topo.center[tgt] - topo.center[src], topo.radius[tgt])
+    sel = want > -4.0
+    assert sel.mean() > 0.3
+    diff = np.abs(got - want)[sel]
+    print(f"[gpu]   {sel.sum()} labels above 1e-4: max |diff| {diff.max():.3e} (log10)", flush=True)
+    assert diff.max() < 0.02
+
+
 def test_mlp_kernels_match_oracle(require_gpu):
     from jaccpot_amd.learned import OrderModel
 
@@ -114,7 +142,7 @@ def test_trained_network_gains_over_fixed_order(require_gpu):
     K = st.topology.num_far_pairs
     plain_flops = m2l_flops(6) * K
     best = None
-    for tol in (-2.0, -1.5, -1.0, -0.5, 0.0):
+    for tol in (-3.0, -2.5, -2.25, -2.0, -1.75, -1.5):
         learned = FastMultipoleMethod(precision="fp32", theta=0.6, softening=1e-3, adaptive_order=True,
                                       p_gears=(2, 4, 6), order_model=model, gear_tolerance=tol)
         stl = learned.prepare_state(x32, m32, leaf_size=32, max_order=6)
@@ -127,7 +155,7 @@ def test_trained_network_gains_over_fixed_order(require_gpu):
     assert best is not None
     e_learned, sched = best
     assert e_learned <= 1.5 * e_plain + 2e-6
-    assert sched.m2l_flops <= 0.8 * plain_flops
+    assert sched.m2l_flops <= 0.85 * plain_flops
     # the network beats an order rule without it (all pairs at the middle gear) on accuracy
     g4 = np.full(K, 4, np.int8)
     from jaccpot_amd.learned import build_gear_schedule
