@@ -44,8 +44,8 @@ WORKLOADS = {
 # leaf size and opening angle are chosen for the lowest time at median force
 # error <= 1e-4 against the FP64 direct sum.
 LEARNED = {
-    "plummer4m_p6": {"leaf": 32, "theta": 0.75, "gears": (2, 4, 6), "gear_tolerance": -1.0},
-    "plummer1m_p4": {"leaf": 32, "theta": 0.75, "gears": (2, 4, 6), "gear_tolerance": -1.0},
+    "plummer4m_p6": {"leaf": 8, "theta": 0.7, "gears": (2, 4, 6), "gear_tolerance": -2.0},
+    "plummer1m_p4": {"leaf": 8, "theta": 0.7, "gears": (2, 4, 6), "gear_tolerance": -2.0},
 }
 
 
@@ -71,8 +71,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="plummer4m_p6", choices=sorted(WORKLOADS))
-    ap.add_argument("--mode", default="learned", choices=("learned", "plain"),
-                    help="learned: time the learned+mixed FMM (value) and the plain one beside it")
+    ap.add_argument("--mode", default="learned", choices=("learned", "plain", "learned-only"),
+                    help="learned: time the learned+mixed FMM (value) and the plain one beside it; "
+                         "learned-only: the learned+mixed FMM alone (profiling)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=16384)
     ap.add_argument("--error-targets", type=int, default=4096)
@@ -105,7 +106,9 @@ def main() -> int:
     gen, N, seed, p, leaf, theta, eps, precision = WORKLOADS[args.workload]
     log(f"workload {args.workload}: {gen} N={N} p={p} leaf={leaf} theta={theta} eps={eps} {precision}")
     x, m = getattr(ics, gen)(N, seed)
-    learned_cfg = LEARNED.get(args.workload) if args.mode == "learned" else None
+    learned_cfg = LEARNED.get(args.workload) if args.mode in ("learned", "learned-only") else None
+    if args.mode == "learned-only" and learned_cfg is None:
+        raise SystemExit(f"no learned+mixed configuration for workload {args.workload}")
 
     def measure(fmm, leaf_size, order, tag):
         """Prepare once, then W warmup + K timed steps with per-stage HIP events."""
@@ -195,8 +198,10 @@ def main() -> int:
         return info
 
     ref_cache = {}
-    plain = measure(FastMultipoleMethod(precision=precision, theta=theta, softening=eps, compensated=True), leaf, p,
-                    "plain")
+    plain = None
+    if args.mode != "learned-only":
+        plain = measure(FastMultipoleMethod(precision=precision, theta=theta, softening=eps, compensated=True), leaf,
+                        p, "plain")
     learned = None
     if learned_cfg is not None:
         from jaccpot_amd.train_order import DEFAULT_WEIGHTS, load
@@ -219,7 +224,8 @@ def main() -> int:
     p2p_s = stage_ms["p2p"] * 1e-3
     achieved = p2p_flops / p2p_s / 1e12
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"p2p_{args.workload}_traffic.json")
+    tag = "learned" if learned is not None else "plain"
+    tfile = os.path.join(ROOT, "profiles", f"p2p_{args.workload}_{tag}_traffic.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
@@ -271,7 +277,7 @@ def main() -> int:
         "stage_ms": stage_ms,
         "plain": plain,
         "learned_mixed": learned,
-        "speedup_vs_plain": (learned["particles_per_s"] / plain["particles_per_s"]) if learned else None,
+        "speedup_vs_plain": (learned["particles_per_s"] / plain["particles_per_s"]) if learned and plain else None,
     }
     print(json.dumps(out), flush=True)
     return 0

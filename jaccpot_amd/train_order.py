@@ -6,7 +6,7 @@ R_d=0.3, z_d=0.03): features of every far pair from the order-6 expansions and
 labels from order-8 translations, both computed by libjfm kernels; the MLP is
 trained with libjfm's backward and Adam kernels.
 
-    python -m jaccpot_amd.train_order [--n 2097152] [--leaf 64] [--steps 2000] [--out PATH]
+    python -m jaccpot_amd.train_order [--n 2097152] [--leaf 16] [--theta 0.7] [--steps 2000] [--out PATH]
 """
 
 from __future__ import annotations
@@ -21,7 +21,7 @@ import numpy as np
 DEFAULT_WEIGHTS = Path(__file__).resolve().parent / "models" / "order_mlp_gears246.npz"
 
 
-def train(n: int = 2_097_152, seed: int = 3, leaf: int = 64, theta: float = 0.6, steps: int = 2000,
+def train(n: int = 2_097_152, seed: int = 3, leaf: int = 16, theta: float = 0.7, steps: int = 2000,
           batch: int = 32768, lr: float = 3e-3, log=print):
     import torch
 
@@ -71,8 +71,8 @@ def load(path: Path = DEFAULT_WEIGHTS):
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, default=2_097_152)
-    ap.add_argument("--leaf", type=int, default=64)
-    ap.add_argument("--theta", type=float, default=0.6)
+    ap.add_argument("--leaf", type=int, default=16)
+    ap.add_argument("--theta", type=float, default=0.7)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--out", type=Path, default=DEFAULT_WEIGHTS)
     a = ap.parse_args(argv)
