@@ -146,6 +146,22 @@ int32_t jfm_m2l_gear(void* stream, int32_t dtype, int32_t p, int32_t gear, int32
                      const int32_t* targets, int32_t num_targets, const int32_t* far_off, const int32_t* far_src,
                      const void* multipoles, const void* centers4, void* locals);
 
+/* M2L over a flat pair list, one lane per pair: locals[pair_tgt[k]] +=
+ * m2l_gear(M[pair_src[k]], c_t - c_s) for k < num_pairs, at order gear <= p
+ * (degrees <= gear accumulated).  Pairs must be grouped by target (CSR order).
+ * scratch holds jfm_m2l_flat_scratch_size(gear, num_pairs) elements of the
+ * working dtype (per-wave partial sums); the span list names every target
+ * whose pairs cross a 64-pair boundary, with the first and last 64-pair block
+ * (index k / 64) holding its pairs.  Deterministic (fixed summation order,
+ * no atomics).  Same operator as jfm_m2l / jfm_m2l_gear (the reference's
+ * runtime/kernels/_m2l.py:1439-1793), with a mapping that keeps every lane busy
+ * when targets have few pairs. */
+int64_t jfm_m2l_flat_scratch_size(int32_t gear, int32_t num_pairs);
+int32_t jfm_m2l_flat(void* stream, int32_t dtype, int32_t p, int32_t gear, int32_t num_pairs,
+                     const int32_t* pair_tgt, const int32_t* pair_src, const void* multipoles,
+                     const void* centers4, void* locals, void* scratch, int32_t num_span,
+                     const int32_t* span_tgt, const int32_t* span_w0, const int32_t* span_w1);
+
 /* One L2L level: locals[c] += l2l(locals[parent[c]], c_parent - c_c) for c in nodes[count].
  * Replaces _propagate_solidfmm_locals_by_level (runtime/kernels/_l2l.py:210-409). */
 int32_t jfm_l2l_level(void* stream, int32_t dtype, int32_t p, const int32_t* nodes, int32_t count,

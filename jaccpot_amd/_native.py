@@ -52,6 +52,9 @@ _SIGNATURES = {
                           c_void_p]),
     "jfm_m2l_gear": (c_int32, [c_void_p, c_int32, c_int32, c_int32, c_int32, c_void_p, c_int32, c_void_p, c_void_p,
                                c_void_p, c_void_p, c_void_p]),
+    "jfm_m2l_flat_scratch_size": (c_int64, [c_int32, c_int32]),
+    "jfm_m2l_flat": (c_int32, [c_void_p, c_int32, c_int32, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
+                               c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p]),
     "jfm_l2l_level": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_int32, c_void_p, c_void_p, c_void_p]),
     "jfm_m2m_top": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p,
                               c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),

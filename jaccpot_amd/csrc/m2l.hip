@@ -70,6 +70,101 @@ __global__ __launch_bounds__(kWave) void m2l_kernel(const int32_t* __restrict__ 
   }
 }
 
+// Flat mapping (jfm_m2l_flat): one wave per 64 consecutive pairs of a CSR
+// far list (pairs grouped by target, pair_tgt[k] = target of pair k); lane l
+// runs pair 64 w + l.  The lanes' contributions go to LDS and lane j sums
+// coefficient j down the 64 rows, closing a sum at every change of target.
+// A target whose pairs lie inside the wave is accumulated into its local
+// directly; one that continues from the previous wave or into the next one
+// leaves a partial sum in scratch (slot 0: the segment holding lane 0 when
+// it continues an earlier wave, slot 1: a segment that starts in this wave
+// and continues), and m2l_fixup_kernel adds those partials in wave order.
+// Deterministic, no atomics; every lane carries a pair whatever the number
+// of pairs per target.
+template <typename T, int P, int G>
+__global__ __launch_bounds__(kWave) void m2l_flat_kernel(int32_t K, const int32_t* __restrict__ pair_tgt,
+                                                         const int32_t* __restrict__ pair_src,
+                                                         const T* __restrict__ mult, const T* __restrict__ centers4,
+                                                         T* __restrict__ locals, int32_t cs,
+                                                         T* __restrict__ scratch) {
+  constexpr int C = ncoef(G);
+  __shared__ T red[kWave][C + 1];
+  __shared__ int32_t tg[kWave];
+  const int lane = threadIdx.x;
+  const int64_t k0 = static_cast<int64_t>(blockIdx.x) * kWave;
+  const int64_t k = k0 + lane;
+  T w[C];
+  int t = -1;
+  if (k < K) {
+    t = pair_tgt[k];
+    const int src = pair_src[k];
+    const V4<T> ct = load4(centers4, t);
+    const V4<T> cs4 = load4(centers4, src);
+    T v[C];
+    load_row<T, C>(mult + static_cast<int64_t>(src) * cs, v);
+    Align<T, G> a;
+    a.init(ct.x - cs4.x, ct.y - cs4.y, ct.z - cs4.z);  // target - source
+    rotate_all<T, G, Rot::ToZMultipole>(v, a);
+    zm2l<T, G>(v, w, a.r);
+    rotate_all<T, G, Rot::FromZLocal>(w, a);
+  } else {
+#pragma unroll
+    for (int j = 0; j < C; ++j) w[j] = T(0);
+  }
+#pragma unroll
+  for (int j = 0; j < C; ++j) red[lane][j] = w[j];
+  // segment ends: the last lane, or a lane whose right neighbour has another target
+  const int tn = __shfl_down(t, 1, kWave);
+  uint64_t ends = __ballot(lane == kWave - 1 || tn != t);
+  tg[lane] = t;
+  __syncthreads();
+  // neighbours of the wave's pair range (uniform)
+  const int before = k0 > 0 ? pair_tgt[k0 - 1] : -1;
+  const int after = k0 + kWave < K ? pair_tgt[k0 + kWave] : -1;
+  T* part = scratch + static_cast<int64_t>(blockIdx.x) * 2 * C;
+  int start = 0;
+  while (ends) {  // scalar loop over the wave's segments
+    const int last = __builtin_ctzll(ends);
+    ends &= ends - 1;
+    const int tl = __builtin_amdgcn_readfirstlane(tg[last]);  // the same for every lane
+    if (tl >= 0) {
+      const bool cont_before = start == 0 && before == tl;
+      const bool cont_after = last == kWave - 1 && after == tl;
+      T* dst = cont_before ? part : cont_after ? part + C : locals + static_cast<int64_t>(tl) * cs;
+      for (int j = lane; j < C; j += kWave) {
+        T s = T(0);
+        for (int l = start; l <= last; ++l) s += red[l][j];
+        if (cont_before || cont_after)
+          dst[j] = s;
+        else
+          dst[j] += s;
+      }
+    }
+    start = last + 1;
+  }
+}
+
+// One wave per target whose pairs span waves w0 < w1 of the flat launch (four
+// targets per workgroup): local += partial(w0, slot 1) + sum_{w0 < w <= w1} partial(w, slot 0).
+template <typename T, int G>
+__global__ __launch_bounds__(4 * kWave) void m2l_fixup_kernel(int32_t num_span, const int32_t* __restrict__ span_tgt,
+                                                              const int32_t* __restrict__ span_w0,
+                                                              const int32_t* __restrict__ span_w1,
+                                                              const T* __restrict__ scratch, T* __restrict__ locals,
+                                                              int32_t cs) {
+  constexpr int C = ncoef(G);
+  const int sp = blockIdx.x * 4 + static_cast<int>(threadIdx.x) / kWave;
+  if (sp >= num_span) return;
+  const int lane = threadIdx.x % kWave;
+  const int t = span_tgt[sp];
+  const int w0 = span_w0[sp], w1 = span_w1[sp];
+  for (int j = lane; j < C; j += kWave) {
+    T s = scratch[static_cast<int64_t>(w0) * 2 * C + C + j];
+    for (int w = w0 + 1; w <= w1; ++w) s += scratch[static_cast<int64_t>(w) * 2 * C + j];
+    locals[static_cast<int64_t>(t) * cs + j] += s;
+  }
+}
+
 }  // namespace
 }  // namespace jfm
 
@@ -127,6 +222,53 @@ extern "C" int32_t jfm_m2l_gear(void* stream, int32_t dtype, int32_t p, int32_t 
   JFM_FOR_DTYPE(dtype, JFM_FOR_ORDER(p, {
     return launch_gear<T, P>(s, gear, accumulate != 0, num_targets, targets, far_off, far_src, multipoles,
                              centers4, locals);
+  }));
+  return JFM_OK;
+}
+
+extern "C" int64_t jfm_m2l_flat_scratch_size(int32_t gear, int32_t num_pairs) {
+  if (gear < 1 || gear > 8 || num_pairs < 0) return -1;
+  return ((static_cast<int64_t>(num_pairs) + 63) / 64) * 2 * ncoef(gear);
+}
+
+namespace {
+template <typename T, int P>
+int32_t launch_flat(hipStream_t s, int32_t gear, int32_t K, const int32_t* pair_tgt, const int32_t* pair_src,
+                    const void* mult, const void* centers4, void* locals, void* scratch, int32_t num_span,
+                    const int32_t* span_tgt, const int32_t* span_w0, const int32_t* span_w1) {
+  const int cs = coef_stride(P, sizeof(T));
+  const int nw = (K + kWave - 1) / kWave;
+#define JFM_FLAT(G)                                                                                              \
+  if (gear == G) {                                                                                               \
+    if constexpr (G <= P) {                                                                                      \
+      m2l_flat_kernel<T, P, G><<<nw, kWave, 0, s>>>(K, pair_tgt, pair_src, (const T*)mult, (const T*)centers4,   \
+                                                    (T*)locals, cs, (T*)scratch);                                \
+      if (num_span > 0)                                                                                          \
+        m2l_fixup_kernel<T, G><<<(num_span + 3) / 4, 4 * kWave, 0, s>>>(num_span, span_tgt, span_w0, span_w1,   \
+                                                                        (const T*)scratch, (T*)locals, cs);      \
+      return check_launch("jfm_m2l_flat");                                                                       \
+    }                                                                                                            \
+  }
+  JFM_FLAT(1) JFM_FLAT(2) JFM_FLAT(3) JFM_FLAT(4) JFM_FLAT(5) JFM_FLAT(6) JFM_FLAT(7) JFM_FLAT(8)
+#undef JFM_FLAT
+  return fail(JFM_ERR_UNSUPPORTED, "jfm_m2l_flat: gear must be in [1, p]");
+}
+}  // namespace
+
+extern "C" int32_t jfm_m2l_flat(void* stream, int32_t dtype, int32_t p, int32_t gear, int32_t num_pairs,
+                                const int32_t* pair_tgt, const int32_t* pair_src, const void* multipoles,
+                                const void* centers4, void* locals, void* scratch, int32_t num_span,
+                                const int32_t* span_tgt, const int32_t* span_w0, const int32_t* span_w1) {
+  if (num_pairs < 0 || num_span < 0) return fail(JFM_ERR_ARG, "jfm_m2l_flat: negative count");
+  if (gear < 1 || gear > p) return fail(JFM_ERR_ARG, "jfm_m2l_flat: gear must be in [1, p]");
+  if (num_pairs == 0) return JFM_OK;
+  if (!pair_tgt || !pair_src || !multipoles || !centers4 || !locals || !scratch)
+    return fail(JFM_ERR_ARG, "jfm_m2l_flat: NULL buffer");
+  if (num_span > 0 && (!span_tgt || !span_w0 || !span_w1)) return fail(JFM_ERR_ARG, "jfm_m2l_flat: NULL span list");
+  hipStream_t s = as_stream(stream);
+  JFM_FOR_DTYPE(dtype, JFM_FOR_ORDER(p, {
+    return launch_flat<T, P>(s, gear, num_pairs, pair_tgt, pair_src, multipoles, centers4, locals, scratch,
+                             num_span, span_tgt, span_w0, span_w1);
   }));
   return JFM_OK;
 }

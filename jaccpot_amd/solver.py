@@ -25,6 +25,7 @@ import numpy as np
 import torch
 
 from . import _native as nat
+from .flatlist import build_flat_list, launch_flat, prefers_flat
 from .topology import Topology, build_topology
 
 _PRESETS = ("fast", "balanced", "accurate", "large_n_gpu")
@@ -240,6 +241,9 @@ class FastMultipoleMethod:
         )
         if self.mixed_precision and dtype == torch.float32:
             state.extra["mixed"] = self._mixed_schedule(topo, p, dev)
+        if prefers_flat(topo.num_far_pairs, int(topo.m2l_targets.size)):
+            state.extra["flat"] = build_flat_list(np.repeat(np.arange(nodes, dtype=np.int32), np.diff(topo.far_off)),
+                                                  topo.far_src, p, dtype, dev)
         xd = torch.as_tensor(x_host, device=dev)
         md = torch.as_tensor(m_host, device=dev)
         if self.adaptive_order:
@@ -346,10 +350,14 @@ class FastMultipoleMethod:
         sched = state.extra.get("gears")
         if sched is not None:
             for i, g in enumerate(sched.gears):
-                if sched.num_targets[i]:
+                if sched.flat[i] is not None:
+                    launch_flat(sched.flat[i], s, code, state.order, state.multipoles, state.centers4, state.locals)
+                elif sched.num_targets[i]:
                     nat.call("jfm_m2l_gear", s, code, state.order, g, 1, sched.targets[i].data_ptr(),
                              sched.num_targets[i], sched.offsets[i].data_ptr(), sched.sources[i].data_ptr(),
                              state.multipoles.data_ptr(), state.centers4.data_ptr(), state.locals.data_ptr())
+        elif state.extra.get("flat") is not None:
+            launch_flat(state.extra["flat"], s, code, state.order, state.multipoles, state.centers4, state.locals)
         elif topo.m2l_targets.size:
             nat.call("jfm_m2l", s, code, state.order, state.m2l_targets.data_ptr(), int(topo.m2l_targets.size),
                      state.far_off.data_ptr(), state.far_src.data_ptr(), state.multipoles.data_ptr(),

@@ -203,3 +203,45 @@ def test_mixed_precision_top_levels(require_gpu):
     ref = ofmm.fmm_accelerations(xr, mr, leaf_size=leaf, p=p, theta=theta, eps=eps, topo=topo)
     rel = np.linalg.norm(acc - ref) / np.linalg.norm(ref)
     assert rel < 1e-5, rel
+
+
+@pytest.mark.parametrize("dtype_name", ["fp64", "fp32"])
+def test_flat_m2l_matches_per_target_m2l(dtype_name, require_gpu):
+    """Lane-per-pair M2L (with targets spanning several 64-pair blocks) equals the
+    wave-per-target kernel, and repeats bit for bit."""
+    from jaccpot_amd import FastMultipoleMethod
+    from jaccpot_amd import _native as nat
+    from jaccpot_amd.flatlist import build_flat_list, launch_flat
+
+    print(f"[gpu] flat m2l {dtype_name}", flush=True)
+    x, m = ics.plummer(6000, 17)
+    if dtype_name == "fp32":
+        x, m = x.astype(np.float32), m.astype(np.float32)
+    fmm = FastMultipoleMethod(precision=dtype_name, theta=0.5, softening=1e-3)
+    state = fmm.prepare_state(x, m, leaf_size=4, max_order=5)
+    topo = state.topology
+    counts = np.diff(topo.far_off)
+    tgt = np.repeat(np.arange(topo.num_nodes, dtype=np.int32), counts)
+    fl = build_flat_list(tgt, topo.far_src, 5, state.dtype, state.device)
+    print(f"[gpu]   {topo.num_far_pairs} pairs, max per target {counts.max()}, {fl.num_span} spanning targets",
+          flush=True)
+    assert fl.num_span > 0 and counts.max() > 128
+    s = nat.current_stream()
+    code = nat.dtype_code(state.dtype)
+    ref = torch.zeros_like(state.locals)
+    nat.call("jfm_m2l", s, code, 5, state.m2l_targets.data_ptr(), int(topo.m2l_targets.size), state.far_off.data_ptr(),
+             state.far_src.data_ptr(), state.multipoles.data_ptr(), state.centers4.data_ptr(), ref.data_ptr())
+    outs = []
+    for _ in range(2):
+        loc = torch.zeros_like(state.locals)
+        launch_flat(fl, s, code, 5, state.multipoles, state.centers4, loc)
+        outs.append(loc)
+    torch.cuda.synchronize()
+    assert torch.equal(outs[0], outs[1])
+    C = 36
+    a = outs[0][:, :C].double().cpu().numpy()
+    b = ref[:, :C].double().cpu().numpy()
+    scale = np.maximum(np.abs(b).max(axis=1, keepdims=True), 1e-300)
+    err = float((np.abs(a - b) / scale).max())
+    print(f"[gpu]   max row-relative difference {err:.3e}", flush=True)
+    assert err < (1e-12 if dtype_name == "fp64" else 2e-6)
