@@ -111,6 +111,73 @@ __global__ __launch_bounds__(kWave) void p2m_kernel(const T* __restrict__ pos4, 
   if (lane == 0) store4(centers4, node, V4<T>{cx, cy, cz, ms});
 }
 
+// Narrow leaves (leaf_size <= G <= 32): G lanes per leaf, 64 / G leaves per
+// wave.  The same per-leaf quantities as p2m_kernel, summed over the leaf's
+// lanes by an xor tree inside each aligned group of G lanes (fixed order).
+template <int G, typename T>
+__device__ __forceinline__ T group_sum(T v) {
+#pragma unroll
+  for (int o = 1; o < G; o <<= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+template <typename T, int P, int G>
+__global__ __launch_bounds__(kWave) void p2m_group_kernel(const T* __restrict__ pos4, int32_t n, int32_t leaf_size,
+                                                          int32_t num_leaves, int32_t num_internal,
+                                                          T* __restrict__ mult, int32_t cs, T* __restrict__ centers4) {
+  constexpr int C = ncoef(P);
+  const int lane = threadIdx.x;
+  const int leaf = blockIdx.x * (kWave / G) + lane / G;
+  const int sub = lane % G;
+  const bool live = leaf < num_leaves;  // dead groups still join the shuffles
+  const int first = leaf * leaf_size;
+  const int cnt = live ? min(leaf_size, n - first) : 0;
+  V4<T> q{T(0), T(0), T(0), T(0)};
+  if (sub < cnt) q = load4(pos4, first + sub);
+  const T ms = group_sum<G>(q.w);
+  const T mx = group_sum<G>(q.w * q.x);
+  const T my = group_sum<G>(q.w * q.y);
+  const T mz = group_sum<G>(q.w * q.z);
+  const T px = group_sum<G>(q.x);
+  const T py = group_sum<G>(q.y);
+  const T pz = group_sum<G>(q.z);
+  T cx, cy, cz;
+  if (ms > T(0)) {
+    cx = mx / ms;
+    cy = my / ms;
+    cz = mz / ms;
+  } else {
+    const T inv = T(1) / static_cast<T>(cnt > 0 ? cnt : 1);
+    cx = px * inv;
+    cy = py * inv;
+    cz = pz * inv;
+  }
+  T acc[C];
+  {
+    T re[ntri(P)], im[ntri(P)];
+    solid_harmonics<T, P>(q.x - cx, q.y - cy, q.z - cz, re, im);
+#pragma unroll
+    for (int nn = 0; nn <= P; ++nn) {
+      acc[sh_idx(nn, 0)] = q.w * re[tri(nn, 0)];
+#pragma unroll
+      for (int m = 1; m <= nn; ++m) {
+        acc[sh_idx(nn, m)] = q.w * re[tri(nn, m)];
+        acc[sh_idx(nn, -m)] = q.w * im[tri(nn, m)];
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < C; ++j) acc[j] = group_sum<G>(acc[j]);
+  if (!live) return;
+  const int node = num_internal + leaf;
+  T* row = mult + static_cast<int64_t>(node) * cs;
+#pragma unroll
+  for (int j = 0; j < C; ++j)
+    if (j % G == sub) row[j] = acc[j];
+  for (int j = C + sub; j < cs; j += G) row[j] = T(0);
+  if (sub == 0) store4(centers4, node, V4<T>{cx, cy, cz, ms});
+}
+
 // One thread per internal node of a level: centre of mass of the two children,
 // M = m2m(M_left, c_left - c) + m2m(M_right, c_right - c).
 template <typename T, int P>
@@ -398,8 +465,18 @@ int32_t jfm_p2m(void* stream, int32_t dtype, int32_t p, const void* pos4, int32_
   hipStream_t s = as_stream(stream);
   JFM_FOR_DTYPE(dtype, JFM_FOR_ORDER(p, {
     const int cs = coef_stride(P, sizeof(T));
-    p2m_kernel<T, P><<<num_leaves, kWave, 0, s>>>((const T*)pos4, n, leaf_size, num_internal, (T*)multipoles, cs,
-                                                  (T*)centers4);
+    if (leaf_size <= 8)
+      p2m_group_kernel<T, P, 8><<<(num_leaves + 7) / 8, kWave, 0, s>>>(
+          (const T*)pos4, n, leaf_size, num_leaves, num_internal, (T*)multipoles, cs, (T*)centers4);
+    else if (leaf_size <= 16)
+      p2m_group_kernel<T, P, 16><<<(num_leaves + 3) / 4, kWave, 0, s>>>(
+          (const T*)pos4, n, leaf_size, num_leaves, num_internal, (T*)multipoles, cs, (T*)centers4);
+    else if (leaf_size <= 32)
+      p2m_group_kernel<T, P, 32><<<(num_leaves + 1) / 2, kWave, 0, s>>>(
+          (const T*)pos4, n, leaf_size, num_leaves, num_internal, (T*)multipoles, cs, (T*)centers4);
+    else
+      p2m_kernel<T, P><<<num_leaves, kWave, 0, s>>>((const T*)pos4, n, leaf_size, num_internal, (T*)multipoles,
+                                                    cs, (T*)centers4);
   }));
   return check_launch("jfm_p2m");
 }
