@@ -6,9 +6,15 @@ gather into tree order, P2M, M2M by level, M2L, L2L by level, L2P and P2P
 in HBM before the timed region; the host topology build (tree + far/near
 lists) happens once in prepare and is reported separately.
 
-Prints ONE JSON line (rank 0).  Default workload: Plummer sphere N=4,194,304
-(a=1, truncated at 10a, masses 1/N, seed 2), float32, p=6, leaf 64,
-theta=0.6, softening 1e-3.
+Each configuration is timed over K eager steps with HIP events per stage
+(the stage breakdown and the P2P roofline); ``--graph`` also times K replays
+of the same step captured once into a HIP graph and reports those as
+``ms_per_step`` (on MI355X the replay measured 3% slower than eager
+launches).  Prints ONE JSON line.  Default workload: Plummer sphere
+N=4,194,304 (a=1, truncated at 10a, masses 1/N, seed 2), float32, softening
+1e-3; ``value`` is the learned+mixed FMM (leaf 8, theta 0.7, learned M2L
+orders, float64 top-level M2M/L2L), and the plain fixed-order FMM at p=6,
+leaf 64, theta=0.6 is timed beside it.
 """
 
 from __future__ import annotations
@@ -74,6 +80,8 @@ def main() -> int:
     ap.add_argument("--mode", default="learned", choices=("learned", "plain", "learned-only"),
                     help="learned: time the learned+mixed FMM (value) and the plain one beside it; "
                          "learned-only: the learned+mixed FMM alone (profiling)")
+    ap.add_argument("--graph", action="store_true",
+                    help="also time the step replayed from a captured HIP graph and report that as ms_per_step")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=16384)
     ap.add_argument("--error-targets", type=int, default=4096)
@@ -169,8 +177,22 @@ def main() -> int:
             stage["l2p"].append(ev[2].elapsed_time(ev[3]))
             stage["p2p"].append(ev[3].elapsed_time(ev[4]))
         stage_ms = {k: float(np.mean(v)) for k, v in stage.items()}
-        log(f"[{tag}] {ms_per_step:.3f} ms/step; stages (ms) "
+        log(f"[{tag}] eager {ms_per_step:.3f} ms/step; stages (ms) "
             f"{json.dumps({k: round(v, 3) for k, v in stage_ms.items()})}")
+        eager_ms = ms_per_step
+        if args.graph:
+            # the same step captured once into a HIP graph and replayed
+            graph = fmm.capture_evaluation(state, xd, md)
+            for _ in range(args.warmup):
+                graph.replay()
+            torch.cuda.synchronize()
+            t_start = time.perf_counter()
+            for _ in range(args.steps):
+                graph.replay()
+            torch.cuda.synchronize()
+            ms_per_step = 1e3 * (time.perf_counter() - t_start) / args.steps
+            log(f"[{tag}] graph replay {ms_per_step:.3f} ms/step")
+            del graph
         err = None
         if args.error_targets > 0:
             rng = np.random.default_rng(12345)
@@ -187,7 +209,8 @@ def main() -> int:
             log(f"[{tag}] force error vs FP64 direct sum: median {err['median']:.3e} p99 {err['p99']:.3e}")
         sched = state.extra.get("gears")
         far_flops = (sched.m2l_flops if sched is not None else m2l_flops(order) * topo.num_far_pairs)
-        info = {"ms_per_step": ms_per_step, "particles_per_s": N / (ms_per_step * 1e-3), "stage_ms": stage_ms,
+        info = {"ms_per_step": ms_per_step, "eager_ms_per_step": eager_ms, "graph": bool(args.graph),
+                "particles_per_s": N / (ms_per_step * 1e-3), "stage_ms": stage_ms,
                 "force_error": err, "prepare_s": prep_s, "topology_build_s": topo.build_seconds,
                 "leaf_size": leaf_size, "far_pairs": topo.num_far_pairs, "m2l_gflop_per_step": far_flops / 1e9,
                 "p2p_interactions": int(topo.p2p_interactions())}

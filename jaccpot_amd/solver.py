@@ -428,6 +428,14 @@ class FastMultipoleMethod:
         out_dtype = state.input_dtype if state.input_dtype in (torch.float32, torch.float64) else t.dtype
         return t.to(device=state.input_device, dtype=out_dtype)
 
+    def capture_evaluation(self, state: FMMPreparedState, positions: torch.Tensor, masses: torch.Tensor,
+                           return_potential: bool = False) -> "EvaluationGraph":
+        """Capture one fixed-topology evaluation (gather, upward, downward, L2P,
+        P2P) into a HIP graph.  ``positions``/``masses`` become the graph's
+        static input buffers (update them in place between replays); the
+        outputs are static tensors in input order."""
+        return EvaluationGraph(self, state, positions, masses, return_potential)
+
     def differentiable_accelerations(self, state: FMMPreparedState, positions, masses, *,
                                      return_potential: bool = False):
         """Accelerations at the state's frozen topology, differentiable w.r.t. positions and masses
@@ -444,3 +452,35 @@ class FastMultipoleMethod:
                                    theta=theta)
         return self.evaluate_prepared_state(state, target_indices=target_indices, return_potential=return_potential,
                                             max_acc_derivative_order=max_acc_derivative_order)
+
+
+class EvaluationGraph:
+    """A captured fixed-topology evaluation; ``replay()`` re-runs it on the
+    current contents of ``positions``/``masses`` and returns (acc[, pot])."""
+
+    def __init__(self, fmm: FastMultipoleMethod, state: FMMPreparedState, positions: torch.Tensor,
+                 masses: torch.Tensor, return_potential: bool = False):
+        if positions.device != state.device or masses.device != state.device:
+            raise ValueError("graph inputs must live on the state's device")
+        self.positions = positions
+        self.masses = masses
+        n = state.num_particles
+        self.acc = torch.empty((n, 3), dtype=state.dtype, device=state.device)
+        self.pot = torch.empty((n,), dtype=state.dtype, device=state.device) if return_potential else None
+
+        def step():
+            fmm.refresh_prepared_state(state, self.positions, self.masses)
+            fmm._evaluate_device(state, self.acc, self.pot)
+
+        side = torch.cuda.Stream(device=state.device)
+        side.wait_stream(torch.cuda.current_stream(state.device))
+        with torch.cuda.stream(side):
+            step()  # eager run on the capture stream first (allocations settle)
+        torch.cuda.current_stream(state.device).wait_stream(side)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            step()
+
+    def replay(self):
+        self.graph.replay()
+        return (self.acc, self.pot) if self.pot is not None else self.acc

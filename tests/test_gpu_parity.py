@@ -245,3 +245,39 @@ def test_flat_m2l_matches_per_target_m2l(dtype_name, require_gpu):
     err = float((np.abs(a - b) / scale).max())
     print(f"[gpu]   max row-relative difference {err:.3e}", flush=True)
     assert err < (1e-12 if dtype_name == "fp64" else 2e-6)
+
+
+@pytest.mark.parametrize("learned", [False, True], ids=["plain", "learned_mixed"])
+def test_graph_replay_matches_eager(learned, require_gpu):
+    """A captured evaluation replays to the eager result, bit for bit, over
+    several replays with eager work and new inputs in between."""
+    from jaccpot_amd import FastMultipoleMethod
+
+    print(f"[gpu] graph replay learned={learned}", flush=True)
+    x, m = ics.plummer(20000, 5)
+    x, m = x.astype(np.float32), m.astype(np.float32)
+    kw = {}
+    if learned:
+        from jaccpot_amd.train_order import load
+
+        kw = dict(mixed_precision=True, adaptive_order=True, p_gears=(2, 4, 6), order_model=load(),
+                  gear_tolerance=-2.0)
+    fmm = FastMultipoleMethod(precision="fp32", theta=0.6, softening=1e-3, **kw)
+    state = fmm.prepare_state(x, m, leaf_size=8, max_order=6)
+    xd = torch.as_tensor(x, device="cuda")
+    md = torch.as_tensor(m, device="cuda")
+    g = fmm.capture_evaluation(state, xd.clone(), md.clone(), return_potential=True)
+    rng = np.random.default_rng(0)
+    for k in range(3):
+        xs = torch.as_tensor(x + rng.normal(scale=1e-4, size=x.shape).astype(np.float32), device="cuda")
+        g.positions.copy_(xs)
+        acc_g, pot_g = g.replay()
+        acc_g, pot_g = acc_g.clone(), pot_g.clone()
+        torch.cuda.synchronize()
+        # eager evaluation of the same inputs, then compare
+        fmm.refresh_prepared_state(state, xs, md)
+        acc_e = torch.empty_like(acc_g)
+        pot_e = torch.empty_like(pot_g)
+        fmm._evaluate_device(state, acc_e, pot_e)
+        torch.cuda.synchronize()
+        assert torch.equal(acc_g, acc_e) and torch.equal(pot_g, pot_e), k
