@@ -12,9 +12,10 @@ of the same step captured once into a HIP graph and reports those as
 ``ms_per_step`` (on MI355X the replay measured 3% slower than eager
 launches).  Prints ONE JSON line.  Default workload: Plummer sphere
 N=4,194,304 (a=1, truncated at 10a, masses 1/N, seed 2), float32, softening
-1e-3; ``value`` is the learned+mixed FMM (leaf 8, theta 0.7, learned M2L
-orders, float64 top-level M2M/L2L), and the plain fixed-order FMM at p=6,
-leaf 64, theta=0.6 is timed beside it.
+1e-3; ``value`` is the learned+mixed FMM (leaf 8, theta 0.85, M2L orders
+3/6/8 chosen per pair by the order network, float64 top-level M2M/L2L), and
+the plain fixed-order FMM at p=6, leaf 64, theta=0.6 and the learned FMM
+with gears 2/4/6 (leaf 8, theta 0.7) are timed beside it.
 """
 
 from __future__ import annotations
@@ -50,9 +51,12 @@ WORKLOADS = {
 # leaf size and opening angle are chosen for the lowest time at median force
 # error <= 1e-4 against the FP64 direct sum.
 LEARNED = {
-    "plummer4m_p6": {"leaf": 8, "theta": 0.7, "gears": (2, 4, 6), "gear_tolerance": -2.0},
-    "plummer1m_p4": {"leaf": 8, "theta": 0.7, "gears": (2, 4, 6), "gear_tolerance": -2.0},
+    "plummer4m_p6": {"leaf": 8, "theta": 0.85, "gears": (3, 6, 8), "gear_tolerance": -1.5},
+    "plummer1m_p4": {"leaf": 8, "theta": 0.85, "gears": (3, 6, 8), "gear_tolerance": -1.5},
 }
+# Reported beside the value: the learned FMM with the gears (2, 4, 6) of the
+# order-model training configuration, at its own fastest setting.
+LEARNED_246 = {"leaf": 8, "theta": 0.7, "gears": (2, 4, 6), "gear_tolerance": -1.5}
 
 
 def log(msg: str) -> None:
@@ -225,17 +229,24 @@ def main() -> int:
     if args.mode != "learned-only":
         plain = measure(FastMultipoleMethod(precision=precision, theta=theta, softening=eps, compensated=True), leaf,
                         p, "plain")
-    learned = None
+    learned = learned246 = None
     if learned_cfg is not None:
-        from jaccpot_amd.train_order import DEFAULT_WEIGHTS, load
+        from jaccpot_amd.train_order import load, weights_path
 
-        if not DEFAULT_WEIGHTS.exists():
-            raise SystemExit(f"order-model weights {DEFAULT_WEIGHTS} missing: run python -m jaccpot_amd.train_order")
-        model = load(DEFAULT_WEIGHTS)
-        lfmm = FastMultipoleMethod(precision=precision, theta=learned_cfg["theta"], softening=eps, compensated=True,
-                                   mixed_precision=True, adaptive_order=True, p_gears=tuple(model.gears),
-                                   order_model=model, gear_tolerance=learned_cfg["gear_tolerance"])
-        learned = measure(lfmm, learned_cfg["leaf"], max(model.gears), "learned")
+        def learned_fmm(cfg):
+            path = weights_path(cfg["gears"])
+            if not path.exists():
+                raise SystemExit(f"order-model weights {path} missing: run python -m jaccpot_amd.train_order "
+                                 f"--gears {','.join(map(str, cfg['gears']))}")
+            model = load(path)
+            return FastMultipoleMethod(precision=precision, theta=cfg["theta"], softening=eps, compensated=True,
+                                       mixed_precision=True, adaptive_order=True, p_gears=tuple(model.gears),
+                                       order_model=model, gear_tolerance=cfg["gear_tolerance"])
+
+        learned = measure(learned_fmm(learned_cfg), learned_cfg["leaf"], max(learned_cfg["gears"]), "learned")
+        if args.mode == "learned":
+            learned246 = measure(learned_fmm(LEARNED_246), LEARNED_246["leaf"], max(LEARNED_246["gears"]),
+                                 "learned246")
     head = learned if learned is not None else plain
     ms_per_step = head["ms_per_step"]
     stage_ms = head["stage_ms"]
@@ -292,7 +303,8 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32" if precision == "fp32" else "f64",
         "data": "synthetic",
-        "config": {"workload": workload, "N": N, "seed": seed, "order": p, "leaf_size": topo_leaf,
+        "config": {"workload": workload, "N": N, "seed": seed,
+                   "order": max(learned_cfg["gears"]) if learned is not None else p, "leaf_size": topo_leaf,
                    "softening": eps, "parallelism": "single-gpu", "mode": mode},
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -300,6 +312,7 @@ def main() -> int:
         "stage_ms": stage_ms,
         "plain": plain,
         "learned_mixed": learned,
+        "learned_mixed_gears246": dict(learned246, config=LEARNED_246) if learned246 else None,
         "speedup_vs_plain": (learned["particles_per_s"] / plain["particles_per_s"]) if learned and plain else None,
     }
     print(json.dumps(out), flush=True)

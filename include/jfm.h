@@ -267,12 +267,15 @@ int32_t jfm_leaf_adjoint_b(void* stream, int32_t dtype, int32_t p, const void* p
  * per far pair of the CSR (far_off [num_nodes+1], far_src), with predictions
  * [K,3] in CSR order and w = log10(m_s / d^2): the smallest of gears3[0..2]
  * with y_g + w <= budget_t + tol_log10, where budget_t is the largest
- * y_top + w over target t's pairs; gears3[2] otherwise. */
+ * y_top + w over target t's pairs; gears3[2] otherwise.  A top gear of 8 is
+ * the label reference (floor label): y_top is then extrapolated,
+ * y_top = y_1 + (g2 - g1) / (g1 - g0) min(y_1 - y_0, 0). */
 int32_t jfm_pair_features(void* stream, int32_t p, int32_t K, const int32_t* tgt_of, const int32_t* far_src,
                           const void* multipoles, const void* centers4, const float* radius, double total_mass,
                           float* features);
 int32_t jfm_pair_labels(void* stream, int32_t K, const int32_t* tgt_of, const int32_t* far_src,
-                        const void* multipoles8, const void* centers4, const float* radius, float* labels);
+                        const void* multipoles8, const void* centers4, const float* radius, const int32_t* gears3,
+                        float* labels);
 int32_t jfm_mlp_num_params(void);
 int64_t jfm_mlp_backward_scratch_floats(int32_t K, int32_t per_block);
 int32_t jfm_mlp_forward(void* stream, int32_t K, const float* X, const uint16_t* wbf16, float* Y);

@@ -25,7 +25,7 @@ EVAL_FAR = 2
 EVAL_POTENTIAL = 4
 EVAL_COMPENSATED = 8
 MAX_ORDER = 8
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 _i32p = POINTER(c_int32)
 _f64p = POINTER(c_double)
@@ -80,7 +80,8 @@ _SIGNATURES = {
                                      c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "jfm_pair_features": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                     c_double, c_void_p]),
-    "jfm_pair_labels": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "jfm_pair_labels": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                  c_void_p]),
     "jfm_mlp_num_params": (c_int32, []),
     "jfm_mlp_backward_scratch_floats": (c_int64, [c_int32, c_int32]),
     "jfm_mlp_forward": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_void_p]),

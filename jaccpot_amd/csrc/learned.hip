@@ -6,7 +6,8 @@
 //   f0 = log2(d / (r_t + r_s))     separation over the summed bounding radii
 //   f1 = log10(m_s / m_total), f2 = log10(m_t / m_total)
 //   f3..f7 = log10(|M_s,n| / (m_s r_s^n) + 1e-30), n = 2..6 (|.| = L2 over m)
-// Label of a pair for gear g: the relative size of the order-g truncation
+// Label of a pair for gear g (any three increasing gears in 1..8; gear 8 is
+// the reference, label = floor): the relative size of the order-g truncation
 // error of the pair's force against the order-8 translation over the target
 // sphere,
 //   e_g = sqrt(sum_n w_n^2 N_n(L8 - Lg)^2) / sqrt(sum_n w_n^2 N_n(L8)^2),
@@ -99,11 +100,32 @@ __device__ __forceinline__ void m2l_order(const float* M, const Align<float, G>&
   rotate_all<float, G, Rot::FromZLocal>(L, a);
 }
 
+// Weighted truncation error of gear G against the order-8 local L8 of one pair.
+template <int G>
+__device__ __forceinline__ float label_error(const float* M, float dx, float dy, float dz, const float* L8,
+                                             const float* w) {
+  float Lg[ncoef(G)];
+  Align<float, G> a;
+  a.init(dx, dy, dz);
+  m2l_order<G>(M, a, Lg);
+  float e = 0.0f;
+  for (int n = 0; n <= 8; ++n)
+    for (int m = -n; m <= n; ++m) {
+      const float lg = (n <= G) ? Lg[sh_idx(n, m)] : 0.0f;
+      const float q = w[n] * degree_norm_root(n, m) * (L8[sh_idx(n, m)] - lg);
+      e += q * q;
+    }
+  return e;
+}
+
+// labels[k][j] for gears g0 < g1 < g2 (each 1..8; gear 8 is the reference
+// itself, so its label is the floor)
 __global__ __launch_bounds__(64) void labels_kernel(int32_t K, const int32_t* __restrict__ tgt_of,
                                                     const int32_t* __restrict__ far_src,
                                                     const float* __restrict__ mult8, int32_t cs8,
                                                     const float* __restrict__ centers4,
-                                                    const float* __restrict__ radius, float* __restrict__ labels) {
+                                                    const float* __restrict__ radius, int32_t g0, int32_t g1,
+                                                    int32_t g2, float* __restrict__ labels) {
   const int k = blockIdx.x * 64 + threadIdx.x;
   if (k >= K) return;
   const int t = tgt_of[k], s = far_src[k];
@@ -132,27 +154,21 @@ __global__ __launch_bounds__(64) void labels_kernel(int32_t K, const int32_t* __
       ref += q * q;
     }
   ref = sqrtf(fmaxf(ref, 1e-37f));
-  float out[3];
-#define JFM_LABEL(IDX, G)                                                             \
-  {                                                                                   \
-    float Lg[ncoef(G)];                                                               \
-    Align<float, G> a;                                                                \
-    a.init(dx, dy, dz);                                                               \
-    m2l_order<G>(M, a, Lg);                                                           \
-    float e = 0.0f;                                                                   \
-    for (int n = 0; n <= 8; ++n)                                                      \
-      for (int m = -n; m <= n; ++m) {                                                 \
-        const float lg = (n <= G) ? Lg[sh_idx(n, m)] : 0.0f;                          \
-        const float q = w[n] * degree_norm_root(n, m) * (L8[sh_idx(n, m)] - lg);      \
-        e += q * q;                                                                   \
-      }                                                                               \
-    out[IDX] = log10f(fmaxf(sqrtf(e) / ref, 1e-7f));                                  \
+  const int gs[3] = {g0, g1, g2};
+  for (int j = 0; j < 3; ++j) {
+    float e = 0.0f;
+    switch (gs[j]) {  // uniform
+      case 1: e = label_error<1>(M, dx, dy, dz, L8, w); break;
+      case 2: e = label_error<2>(M, dx, dy, dz, L8, w); break;
+      case 3: e = label_error<3>(M, dx, dy, dz, L8, w); break;
+      case 4: e = label_error<4>(M, dx, dy, dz, L8, w); break;
+      case 5: e = label_error<5>(M, dx, dy, dz, L8, w); break;
+      case 6: e = label_error<6>(M, dx, dy, dz, L8, w); break;
+      case 7: e = label_error<7>(M, dx, dy, dz, L8, w); break;
+      default: e = 0.0f; break;
+    }
+    labels[static_cast<int64_t>(k) * 3 + j] = log10f(fmaxf(sqrtf(e) / ref, 1e-7f));
   }
-  JFM_LABEL(0, 2)
-  JFM_LABEL(1, 4)
-  JFM_LABEL(2, 6)
-#undef JFM_LABEL
-  for (int j = 0; j < 3; ++j) labels[static_cast<int64_t>(k) * 3 + j] = out[j];
 }
 
 // --------------------------------------------------------------------- MLP
@@ -333,7 +349,8 @@ __global__ __launch_bounds__(kThreads) void to_bf16_kernel(int32_t n, const floa
 // Per target node t (one thread), over its far pairs k in [far_off[t], far_off[t+1]):
 // w_k = log10(m_s / d^2) scales a pair's predicted relative error y_k,g to an
 // absolute force-error proxy.  The budget of t is the largest proxy of the
-// top gear over its pairs (the fixed-order method's dominant error term);
+// top gear over its pairs (the fixed-order method's dominant error term;
+// for a top gear of 8, y_8 = y_g1 + (8 - g1) / (g1 - g0) (y_g1 - y_g0));
 // pair k takes the smallest gear with y_k,g + w_k <= budget + tol, the top
 // gear otherwise.
 __global__ __launch_bounds__(kThreads) void select_gears_kernel(int32_t num_nodes, const int32_t* __restrict__ far_off,
@@ -353,8 +370,17 @@ __global__ __launch_bounds__(kThreads) void select_gears_kernel(int32_t num_node
     const float d2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-37f);
     return log10f(fmaxf(cs4.w, 1e-37f) / d2);
   };
+  // the top gear's predicted error; a top gear of 8 is the label reference
+  // (its label is the floor), so its error is extrapolated geometrically in
+  // the order from the two lower gears
+  const bool extrap = g2 >= 8;
+  const float slope = static_cast<float>(g2 - g1) / static_cast<float>(g1 - g0);
+  auto top = [&](int k) {
+    const float* y = Y + static_cast<int64_t>(k) * kOut;
+    return extrap ? y[1] + slope * fminf(y[1] - y[0], 0.0f) : y[2];
+  };
   float budget = -INFINITY;
-  for (int k = lo; k < hi; ++k) budget = fmaxf(budget, Y[static_cast<int64_t>(k) * kOut + 2] + weight(k));
+  for (int k = lo; k < hi; ++k) budget = fmaxf(budget, top(k) + weight(k));
   for (int k = lo; k < hi; ++k) {
     const float thr = budget + tol - weight(k);
     const float* y = Y + static_cast<int64_t>(k) * kOut;
@@ -398,14 +424,17 @@ int32_t jfm_pair_features(void* stream, int32_t p, int32_t K, const int32_t* tgt
 }
 
 int32_t jfm_pair_labels(void* stream, int32_t K, const int32_t* tgt_of, const int32_t* far_src,
-                        const void* multipoles8, const void* centers4, const float* radius, float* labels) {
-  if (K < 0) return fail(JFM_ERR_ARG, "jfm_pair_labels: bad sizes");
+                        const void* multipoles8, const void* centers4, const float* radius, const int32_t* gears3,
+                        float* labels) {
+  if (K < 0 || !gears3) return fail(JFM_ERR_ARG, "jfm_pair_labels: bad arguments");
+  if (!(1 <= gears3[0] && gears3[0] < gears3[1] && gears3[1] < gears3[2] && gears3[2] <= 8))
+    return fail(JFM_ERR_ARG, "jfm_pair_labels: gears must be increasing within 1..8");
   if (K == 0) return JFM_OK;
   if (!tgt_of || !far_src || !multipoles8 || !centers4 || !radius || !labels)
     return fail(JFM_ERR_ARG, "jfm_pair_labels: NULL buffer");
   labels_kernel<<<nblocks(K, 64), 64, 0, as_stream(stream)>>>(K, tgt_of, far_src, (const float*)multipoles8,
                                                                coef_stride(8, 4), (const float*)centers4, radius,
-                                                               labels);
+                                                               gears3[0], gears3[1], gears3[2], labels);
   return check_launch("jfm_pair_labels");
 }
 

@@ -18,11 +18,13 @@ from . import _native as nat
 BLOCK = 64  # pairs per wave in jfm_m2l_flat
 
 # A far list runs lane-per-pair when its targets average fewer pairs than
-# this: the wave-per-target kernel then idles most lanes, while for denser
+# this: the wave-per-target kernel then idles many lanes, while for denser
 # lists it keeps its lanes busy and needs no cross-wave fix-up (measured on
-# MI355X: gear-2/4 lists with ~10 pairs per target run 2x faster flat, the
-# gear-6 list with ~80 pairs per target 4% slower plus the fix-up pass).
-FLAT_MAX_MEAN_PAIRS = 32
+# MI355X: gear-2/4 lists with ~10 pairs per target run 2x faster flat, a
+# gear-8 list with ~36 pairs per target costs 15% more per pair than the
+# same gear with ~57, the gear-6 list with ~80 pairs per target runs 4%
+# slower flat plus the fix-up pass).
+FLAT_MAX_MEAN_PAIRS = 56
 
 
 def prefers_flat(num_pairs: int, num_targets: int) -> bool:

@@ -40,7 +40,8 @@ def test_gear_m2l_matches_oracle(require_gpu):
     assert float((np.abs(gl - Lc) / scale).max()) < 1e-12
 
 
-def test_pair_labels_match_oracle(require_gpu):
+@pytest.mark.parametrize("gears", [(2, 4, 6), (3, 6, 8)], ids=["g246", "g368"])
+def test_pair_labels_match_oracle(gears, require_gpu):
     """GPU labels (float32, order-8 upward on the device) against the float64
     label oracle on the same topology, where the error is above float32 noise."""
     from jaccpot_amd import FastMultipoleMethod
@@ -52,7 +53,7 @@ def test_pair_labels_match_oracle(require_gpu):
     leaf, theta = 16, 0.6
     fmm = FastMultipoleMethod(precision="fp32", theta=theta, softening=1e-3)
     state = fmm.prepare_state(x.astype(np.float32), m.astype(np.float32), leaf_size=leaf, max_order=6)
-    got = pair_labels(state).cpu().numpy().astype(np.float64)
+    got = pair_labels(state, gears).cpu().numpy().astype(np.float64)
     x32 = x.astype(np.float32).astype(np.float64)
     m32 = m.astype(np.float32).astype(np.float64)
     topo = OT.build_topology(x32, m32, leaf, theta)
@@ -60,9 +61,11 @@ def test_pair_labels_match_oracle(require_gpu):
     M8 = ofmm.upward(topo, x32[topo.perm], m32[topo.perm], 8)
     tgt = np.repeat(np.arange(topo.num_nodes), np.diff(topo.far_off))
     src = topo.far_src
-    want = OL.pair_labels(M8[src], topo.center[tgt] - topo.center[src], topo.radius[tgt])
+    want = OL.pair_labels(M8[src], topo.center[tgt] - topo.center[src], topo.radius[tgt], gears=gears)
+    if gears[-1] == 8:
+        assert np.abs(got[:, -1] - np.log10(OL.LABEL_FLOOR)).max() < 1e-5
     sel = want > -4.0
-    assert sel.mean() > 0.3
+    assert sel.mean() > 0.2
     diff = np.abs(got - want)[sel]
     print(f"[gpu]   {sel.sum()} labels above 1e-4: max |diff| {diff.max():.3e} (log10)", flush=True)
     assert diff.max() < 0.02
