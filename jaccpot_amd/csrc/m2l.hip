@@ -20,6 +20,9 @@ namespace jfm {
 namespace {
 
 constexpr int kWave = 64;
+// coefficients per pass of the LDS reductions: a 64 x 33 buffer (8.4 KiB for
+// float32) keeps LDS from limiting the waves resident per CU
+constexpr int kChunk = 32;
 
 // G <= P is the pair's expansion order (gear): the source multipole is read
 // to degree G, translated at order G and added into degrees <= G of the
@@ -32,7 +35,7 @@ __global__ __launch_bounds__(kWave) void m2l_kernel(const int32_t* __restrict__ 
                                                     const T* __restrict__ centers4, T* __restrict__ locals,
                                                     int32_t cs) {
   constexpr int C = ncoef(G);
-  __shared__ T red[kWave][C + 1];
+  __shared__ T red[kWave][kChunk + 1];
   const int tgt = targets[blockIdx.x];
   const int lane = threadIdx.x;
   const int b = far_off[tgt], e = far_off[tgt + 1];
@@ -54,20 +57,26 @@ __global__ __launch_bounds__(kWave) void m2l_kernel(const int32_t* __restrict__ 
 #pragma unroll
     for (int j = 0; j < C; ++j) acc[j] += w[j];
   }
+  // lane partial sums reduced through LDS, kChunk coefficients at a time
+  T* row = locals + static_cast<int64_t>(tgt) * cs;
 #pragma unroll
-  for (int j = 0; j < C; ++j) red[lane][j] = acc[j];
-  __syncthreads();
-  const int lim = ACC ? C : cs;
-  for (int j = lane; j < lim; j += kWave) {
-    T s = T(0);
-    if (j < C)
-      for (int l = 0; l < kWave; ++l) s += red[l][j];
-    T* dst = locals + static_cast<int64_t>(tgt) * cs + j;
-    if (ACC)
-      *dst += s;
-    else
-      *dst = s;
+  for (int c0 = 0; c0 < C; c0 += kChunk) {
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j)
+      if (c0 + j < C) red[lane][j] = acc[c0 + j];
+    __syncthreads();
+    if (lane < kChunk && c0 + lane < C) {
+      T s = T(0);
+      for (int l = 0; l < kWave; ++l) s += red[l][lane];
+      if (ACC)
+        row[c0 + lane] += s;
+      else
+        row[c0 + lane] = s;
+    }
   }
+  if (!ACC)
+    for (int j = C + lane; j < cs; j += kWave) row[j] = T(0);
 }
 
 // Flat mapping (jfm_m2l_flat): one wave per 64 consecutive pairs of a CSR
@@ -88,7 +97,7 @@ __global__ __launch_bounds__(kWave) void m2l_flat_kernel(int32_t K, const int32_
                                                          T* __restrict__ locals, int32_t cs,
                                                          T* __restrict__ scratch) {
   constexpr int C = ncoef(G);
-  __shared__ T red[kWave][C + 1];
+  __shared__ T red[kWave][kChunk + 1];
   __shared__ int32_t tg[kWave];
   const int lane = threadIdx.x;
   const int64_t k0 = static_cast<int64_t>(blockIdx.x) * kWave;
@@ -111,36 +120,43 @@ __global__ __launch_bounds__(kWave) void m2l_flat_kernel(int32_t K, const int32_
 #pragma unroll
     for (int j = 0; j < C; ++j) w[j] = T(0);
   }
-#pragma unroll
-  for (int j = 0; j < C; ++j) red[lane][j] = w[j];
   // segment ends: the last lane, or a lane whose right neighbour has another target
   const int tn = __shfl_down(t, 1, kWave);
-  uint64_t ends = __ballot(lane == kWave - 1 || tn != t);
+  const uint64_t ends0 = __ballot(lane == kWave - 1 || tn != t);
   tg[lane] = t;
-  __syncthreads();
   // neighbours of the wave's pair range (uniform)
   const int before = k0 > 0 ? pair_tgt[k0 - 1] : -1;
   const int after = k0 + kWave < K ? pair_tgt[k0 + kWave] : -1;
   T* part = scratch + static_cast<int64_t>(blockIdx.x) * 2 * C;
-  int start = 0;
-  while (ends) {  // scalar loop over the wave's segments
-    const int last = __builtin_ctzll(ends);
-    ends &= ends - 1;
-    const int tl = __builtin_amdgcn_readfirstlane(tg[last]);  // the same for every lane
-    if (tl >= 0) {
-      const bool cont_before = start == 0 && before == tl;
-      const bool cont_after = last == kWave - 1 && after == tl;
-      T* dst = cont_before ? part : cont_after ? part + C : locals + static_cast<int64_t>(tl) * cs;
-      for (int j = lane; j < C; j += kWave) {
+  // the lanes' contributions go through LDS kChunk coefficients at a time
+#pragma unroll
+  for (int c0 = 0; c0 < C; c0 += kChunk) {
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j)
+      if (c0 + j < C) red[lane][j] = w[c0 + j];
+    __syncthreads();
+    const int j = c0 + lane;
+    uint64_t ends = ends0;
+    int start = 0;
+    while (ends) {  // scalar loop over the wave's segments
+      const int last = __builtin_ctzll(ends);
+      ends &= ends - 1;
+      const int tl = __builtin_amdgcn_readfirstlane(tg[last]);  // the same for every lane
+      if (tl >= 0 && lane < kChunk && j < C) {
+        const bool cont_before = start == 0 && before == tl;
+        const bool cont_after = last == kWave - 1 && after == tl;
         T s = T(0);
-        for (int l = start; l <= last; ++l) s += red[l][j];
-        if (cont_before || cont_after)
-          dst[j] = s;
+        for (int l = start; l <= last; ++l) s += red[l][lane];
+        if (cont_before)
+          part[j] = s;
+        else if (cont_after)
+          part[C + j] = s;
         else
-          dst[j] += s;
+          locals[static_cast<int64_t>(tl) * cs + j] += s;
       }
+      start = last + 1;
     }
-    start = last + 1;
   }
 }
 
