@@ -86,6 +86,7 @@ def main() -> int:
                          "learned-only: the learned+mixed FMM alone (profiling)")
     ap.add_argument("--graph", action="store_true",
                     help="also time the step replayed from a captured HIP graph and report that as ms_per_step")
+    ap.add_argument("--no-grad", action="store_true", help="skip the differentiable-path timing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=16384)
     ap.add_argument("--error-targets", type=int, default=4096)
@@ -269,6 +270,10 @@ def main() -> int:
                 "traffic": traffic, "interactions_per_launch": int(interactions), "flops_per_interaction": 20,
                 "avg_launch_ms": round(stage_ms["p2p"], 4)}
 
+    grad = None
+    if args.mode == "learned" and not args.no_grad:
+        grad = measure_gradient(torch, FastMultipoleMethod, ics, args)
+
     cpu = None
     if not args.no_cpu_baseline:
         cn = min(args.cpu_sample, N)
@@ -313,10 +318,56 @@ def main() -> int:
         "plain": plain,
         "learned_mixed": learned,
         "learned_mixed_gears246": dict(learned246, config=LEARNED_246) if learned246 else None,
+        "differentiable": grad,
         "speedup_vs_plain": (learned["particles_per_s"] / plain["particles_per_s"]) if learned and plain else None,
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def measure_gradient(torch, FastMultipoleMethod, ics, args):
+    """Forward + backward of the differentiable FMM on 64 Gaussian blobs
+    (N=1,048,576, sigma 0.05, lognormal masses, seed 4), float32, p=4, leaf 32,
+    theta 0.6, softening 1e-3; loss = sum_t m_t phi_t + 0.5 |a_t|^2 over 1024
+    seeded targets; gradients with respect to all positions and masses."""
+    n = 1_048_576
+    x, m = ics.blobs(n, 4)
+    fmm = FastMultipoleMethod(precision="fp32", theta=0.6, softening=1e-3)
+    t0 = time.perf_counter()
+    state = fmm.prepare_state(x.astype(np.float32), m.astype(np.float32), leaf_size=32, max_order=4)
+    torch.cuda.synchronize()
+    prep = time.perf_counter() - t0
+    tgt = torch.as_tensor(np.sort(np.random.default_rng(12345).choice(n, 1024, replace=False)), device="cuda")
+    xt = torch.as_tensor(x, dtype=torch.float32, device="cuda").requires_grad_(True)
+    mt = torch.as_tensor(m, dtype=torch.float32, device="cuda").requires_grad_(True)
+
+    def step(backward: bool):
+        acc, pot = fmm.differentiable_accelerations(state, xt, mt, return_potential=True)
+        loss = (mt[tgt] * pot[tgt]).sum() + 0.5 * (acc[tgt] * acc[tgt]).sum()
+        if backward:
+            xt.grad = None
+            mt.grad = None
+            loss.backward()
+        return loss
+
+    for _ in range(max(1, args.warmup)):
+        step(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(False)
+    torch.cuda.synchronize()
+    fwd = (time.perf_counter() - t0) / args.steps
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(True)
+    torch.cuda.synchronize()
+    both = (time.perf_counter() - t0) / args.steps
+    log(f"[grad] blobs N={n} p=4 leaf=32: forward {1e3 * fwd:.2f} ms, forward+backward {1e3 * both:.2f} ms")
+    return {"workload": "64 Gaussian blobs N=1048576 sigma 0.05, lognormal masses, seed 4, float32, p=4, leaf 32, "
+                        "theta 0.6, softening 1e-3; loss over 1024 targets",
+            "forward_ms": 1e3 * fwd, "forward_backward_ms": 1e3 * both, "prepare_s": prep,
+            "particles_per_s_forward_backward": n / both}
 
 
 def platform_cpu() -> str:
