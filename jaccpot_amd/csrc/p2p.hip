@@ -9,7 +9,9 @@
 // tile partition q takes slots q, q + NPART, ... so every lane of a wave reads
 // one broadcast address per interaction and the tail tile stays balanced.
 // The next tile's sources are fetched into registers while the current one is
-// consumed.
+// consumed, with the near-list entries loaded one tile further ahead than
+// the positions they address (no load is waited for before a tile's
+// interactions).
 // Interactions use the hardware v_rsq_f32 (float32); per-tile partial sums are
 // folded into the running total with Kahan compensation when requested; the
 // partitions are reduced through LDS in fixed order (deterministic).
@@ -92,6 +94,51 @@ __device__ __forceinline__ void fetch_sources(const T* __restrict__ pos4, int n,
   }
 }
 
+// Two-stage form of fetch_sources: fetch_list loads the raw near-list
+// entries of a tile's slots (no arithmetic on them, so nothing waits for
+// them here), fetch_positions decodes the same slots with those entries one
+// tile later and loads the positions.  The P2P loop keeps list loads one
+// tile ahead of position loads, so neither is waited for before the
+// interactions of the current tile.
+template <int WT, int LOADS, int STRIDE>
+__device__ __forceinline__ void slot_decode(int leaf_size, int slot, int& qi, int& o) {
+  if (leaf_size == WT) {
+    qi = static_cast<int>(static_cast<unsigned>(slot) / WT);
+    o = static_cast<int>(static_cast<unsigned>(slot) % WT);
+  } else {
+    qi = slot / leaf_size;
+    o = slot - qi * leaf_size;
+  }
+}
+
+template <int WT, int LOADS, int STRIDE>
+__device__ __forceinline__ void fetch_list(int leaf_size, const int32_t* __restrict__ near_list, int nnear,
+                                           const int32_t* __restrict__ fallback, int first, int (&v)[LOADS]) {
+  const int32_t* list = nnear > 0 ? near_list : fallback;
+  const int hi = max(nnear - 1, 0);
+#pragma unroll
+  for (int l = 0; l < LOADS; ++l) {
+    int qi, o;
+    slot_decode<WT, LOADS, STRIDE>(leaf_size, first + l * STRIDE, qi, o);
+    v[l] = list[min(max(qi - 1, 0), hi)];
+  }
+}
+
+template <typename T, int WT, int LOADS, int STRIDE>
+__device__ __forceinline__ void fetch_positions(const T* __restrict__ pos4, int n, int leaf_size, int leaf, int total,
+                                                int first, const int (&v)[LOADS], V4<T> (&out)[LOADS],
+                                                bool (&valid)[LOADS]) {
+#pragma unroll
+  for (int l = 0; l < LOADS; ++l) {
+    int qi, o;
+    slot_decode<WT, LOADS, STRIDE>(leaf_size, first + l * STRIDE, qi, o);
+    const int sl = qi > 0 ? v[l] : leaf;
+    const int g = first + l * STRIDE < total ? sl * leaf_size + o : n;
+    valid[l] = g < n;
+    out[l] = load4(pos4, min(g, n - 1));
+  }
+}
+
 // SAFE: guard r^2 == 0 (unsoftened self term / coincident points).
 template <typename T, int WT, bool POT, bool COMP, bool SAFE>
 __global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos4, int32_t n, int32_t leaf_size,
@@ -124,8 +171,11 @@ __global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos
   // interaction loop.
   V4<T> next[LOADS];
   bool ok[LOADS];
-  fetch_sources<T, WT, LOADS, kThreads>(pos4, n, leaf_size, leaf, near_src + nb, nsl - 1, near_off, total,
-                                        threadIdx.x, next, ok);
+  int lst[LOADS];  // near-list entries of the tile after `next`
+  const int tid = threadIdx.x;
+  fetch_list<WT, LOADS, kThreads>(leaf_size, near_src + nb, nsl - 1, near_off, tid, lst);
+  fetch_positions<T, WT, LOADS, kThreads>(pos4, n, leaf_size, leaf, total, tid, lst, next, ok);
+  fetch_list<WT, LOADS, kThreads>(leaf_size, near_src + nb, nsl - 1, near_off, TILE + tid, lst);
 
   T sx = 0, sy = 0, sz = 0, sp = 0;
   T kx = 0, ky = 0, kz = 0, kp = 0;
@@ -138,10 +188,10 @@ __global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos
       tile[l * kThreads + threadIdx.x] = q;
     }
     __syncthreads();
-    // unconditional (past the end it yields invalid slots): a branch here
+    // unconditional (past the end they yield invalid slots): a branch here
     // would make the compiler wait for the loads at its join
-    fetch_sources<T, WT, LOADS, kThreads>(pos4, n, leaf_size, leaf, near_src + nb, nsl - 1, near_off, total,
-                                          base + TILE + threadIdx.x, next, ok);
+    fetch_positions<T, WT, LOADS, kThreads>(pos4, n, leaf_size, leaf, total, base + TILE + tid, lst, next, ok);
+    fetch_list<WT, LOADS, kThreads>(leaf_size, near_src + nb, nsl - 1, near_off, base + 2 * TILE + tid, lst);
     // Slots past `total` hold zero-mass sources, so every partition runs the
     // same (wave-uniform, scalar-controlled) trip count.
     const int trips = (min(TILE, total - base) + NPART - 1) / NPART;
