@@ -50,56 +50,18 @@ __device__ __forceinline__ void kahan_add(T& s, T& c, T v) {
   s = t;
 }
 
-// Sources of slots first + l * STRIDE (l < LOADS) of a target leaf's slot
-// sequence (self leaf, then its nnear near leaves, leaf_size slots each) into
-// registers; valid[l] is false for slots past `total` or past the last
-// particle (the caller stores those as zero-mass sources).  The loads are
-// unconditional: indices are clamped into range (an empty near list reads
-// `fallback`, a valid int32 location, and discards the value), so all
-// near-list loads are in flight together, then all position loads, which stay
-// in flight until the caller stores them.  For leaf_size == WT (power-of-two
+// Source fetch in two stages.  A target leaf's slot sequence is its own
+// leaf, then its nnear near leaves, leaf_size slots each; thread t owns slots
+// first + l * STRIDE (l < LOADS).  fetch_list loads the raw near-list entries
+// of those slots (indices clamped into range; an empty list reads
+// `fallback`, a valid int32 location, and the value is discarded; no
+// arithmetic on the loaded values, so nothing waits for them here);
+// fetch_positions decodes the same slots with those entries one tile later
+// and loads the positions (slots past `total` or past the last particle are
+// marked invalid and become zero-mass sources).  The P2P loop keeps list
+// loads one tile ahead of position loads, so neither is waited for before the
+// interactions of the current tile.  For leaf_size == WT (power-of-two
 // leaves) the slot decode is a shift and a mask.
-template <typename T, int WT, int LOADS, int STRIDE>
-__device__ __forceinline__ void fetch_sources(const T* __restrict__ pos4, int n, int leaf_size, int leaf,
-                                              const int32_t* __restrict__ near_list, int nnear,
-                                              const int32_t* __restrict__ fallback, int total, int first,
-                                              V4<T> (&out)[LOADS], bool (&valid)[LOADS]) {
-  int qi[LOADS], o[LOADS];
-  if (leaf_size == WT) {
-#pragma unroll
-    for (int l = 0; l < LOADS; ++l) {
-      const unsigned slot = static_cast<unsigned>(first + l * STRIDE);
-      qi[l] = static_cast<int>(slot / WT);
-      o[l] = static_cast<int>(slot % WT);
-    }
-  } else {
-#pragma unroll
-    for (int l = 0; l < LOADS; ++l) {
-      const int slot = first + l * STRIDE;
-      qi[l] = slot / leaf_size;
-      o[l] = slot - qi[l] * leaf_size;
-    }
-  }
-  const int32_t* list = nnear > 0 ? near_list : fallback;
-  const int hi = max(nnear - 1, 0);
-  int v[LOADS];
-#pragma unroll
-  for (int l = 0; l < LOADS; ++l) v[l] = list[min(max(qi[l] - 1, 0), hi)];
-#pragma unroll
-  for (int l = 0; l < LOADS; ++l) {
-    const int sl = qi[l] > 0 ? v[l] : leaf;
-    const int g = first + l * STRIDE < total ? sl * leaf_size + o[l] : n;
-    valid[l] = g < n;
-    out[l] = load4(pos4, min(g, n - 1));
-  }
-}
-
-// Two-stage form of fetch_sources: fetch_list loads the raw near-list
-// entries of a tile's slots (no arithmetic on them, so nothing waits for
-// them here), fetch_positions decodes the same slots with those entries one
-// tile later and loads the positions.  The P2P loop keeps list loads one
-// tile ahead of position loads, so neither is waited for before the
-// interactions of the current tile.
 template <int WT, int LOADS, int STRIDE>
 __device__ __forceinline__ void slot_decode(int leaf_size, int slot, int& qi, int& o) {
   if (leaf_size == WT) {
