@@ -170,3 +170,48 @@ def test_trained_network_gains_over_fixed_order(require_gpu):
     e_g4 = err(plain.evaluate_prepared_state(st))
     print(f"[gpu]   all pairs at gear 4: err {e_g4:.3e}", flush=True)
     assert e_learned < e_g4
+
+
+def test_shipped_gear368_model_cuts_order8_work(require_gpu):
+    """The shipped (3, 6, 8) network (top gear = the labels' order-8 reference,
+    budget extrapolated from the two lower gears) keeps the force error of the
+    all-order-8 method within 1.5x while sending a share of the far pairs to
+    orders 6 and 3, on a configuration it was not trained on (Plummer)."""
+    from jaccpot_amd import FastMultipoleMethod, direct_sum_gravitational_acceleration
+    from jaccpot_amd.learned import build_gear_schedule
+    from jaccpot_amd.perf import m2l_flops
+    from jaccpot_amd.train_order import load, weights_path
+
+    print("[gpu] shipped (3,6,8) model", flush=True)
+    model = load(weights_path((3, 6, 8)))
+    x, m = ics.plummer(131072, 8)
+    x32, m32 = x.astype(np.float32), m.astype(np.float32)
+    tgt = np.random.default_rng(12345).choice(x.shape[0], 2048, replace=False)
+    ref = direct_sum_gravitational_acceleration(x32.astype(np.float64), m32.astype(np.float64), softening=1e-3,
+                                                target_indices=tgt).cpu().numpy()
+
+    def err(acc):
+        return float(np.median(np.linalg.norm(acc[tgt] - ref, axis=1) / np.linalg.norm(ref, axis=1)))
+
+    plain = FastMultipoleMethod(precision="fp32", theta=0.85, softening=1e-3)
+    st = plain.prepare_state(x32, m32, leaf_size=8, max_order=8)
+    e8 = err(plain.evaluate_prepared_state(st))
+    K = st.topology.num_far_pairs
+    learned = FastMultipoleMethod(precision="fp32", theta=0.85, softening=1e-3, adaptive_order=True,
+                                  p_gears=(3, 6, 8), order_model=model, gear_tolerance=-1.5)
+    stl = learned.prepare_state(x32, m32, leaf_size=8, max_order=8)
+    el = err(learned.evaluate_prepared_state(stl))
+    sched = stl.extra["gears"]
+    frac = sched.m2l_flops / (m2l_flops(8) * K)
+    print(f"[gpu]   order 8: err {e8:.3e}; learned: err {el:.3e}, gears {sched.counts}, M2L flops {frac:.3f}",
+          flush=True)
+    assert el <= 1.5 * e8 + 2e-6
+    assert frac <= 0.9
+    # without the extrapolated budget (every pair at gear 6) the error is far larger
+    st.extra["gears"] = build_gear_schedule(st, np.full(K, 6, np.int8), (3, 6, 8))
+    from jaccpot_amd import _native as nat
+
+    plain._downward(st, nat.current_stream(), nat.JFM_F32)
+    e6 = err(plain.evaluate_prepared_state(st))
+    print(f"[gpu]   all pairs at order 6: err {e6:.3e}", flush=True)
+    assert el < e6
