@@ -263,7 +263,9 @@ int32_t jfm_leaf_adjoint_b(void* stream, int32_t dtype, int32_t p, const void* p
  * float32 accumulation.  jfm_mlp_backward: gradient of the mean squared error
  * over the K pairs (grad_and_loss[num_params] = loss), deterministic; scratch
  * holds jfm_mlp_backward_scratch_floats(K, per_block) floats.  jfm_mlp_adam:
- * one Adam step on the float32 masters, refreshing wbf16.  jfm_select_gears:
+ * one Adam step on the float32 masters, refreshing wbf16.  jfm_mlp_forward
+ * normalises the features on the fly when mean / inv_std [8] are given
+ * (both NULL: X is already normalised).  jfm_select_gears:
  * per far pair of the CSR (far_off [num_nodes+1], far_src), with predictions
  * [K,3] in CSR order and w = log10(m_s / d^2): the smallest of gears3[0..2]
  * with y_g + w <= budget_t + tol_log10, where budget_t is the largest
@@ -278,7 +280,8 @@ int32_t jfm_pair_labels(void* stream, int32_t K, const int32_t* tgt_of, const in
                         float* labels);
 int32_t jfm_mlp_num_params(void);
 int64_t jfm_mlp_backward_scratch_floats(int32_t K, int32_t per_block);
-int32_t jfm_mlp_forward(void* stream, int32_t K, const float* X, const uint16_t* wbf16, float* Y);
+int32_t jfm_mlp_forward(void* stream, int32_t K, const float* X, const float* mean, const float* inv_std,
+                        const uint16_t* wbf16, float* Y);
 int32_t jfm_mlp_backward(void* stream, int32_t K, int32_t per_block, const float* X, const float* targets,
                          const uint16_t* wbf16, float* scratch, float* grad_and_loss);
 int32_t jfm_mlp_adam(void* stream, int32_t step, float lr, float beta1, float beta2, float eps, float* master,

@@ -25,7 +25,7 @@ EVAL_FAR = 2
 EVAL_POTENTIAL = 4
 EVAL_COMPENSATED = 8
 MAX_ORDER = 8
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 _i32p = POINTER(c_int32)
 _f64p = POINTER(c_double)
@@ -84,7 +84,7 @@ _SIGNATURES = {
                                   c_void_p]),
     "jfm_mlp_num_params": (c_int32, []),
     "jfm_mlp_backward_scratch_floats": (c_int64, [c_int32, c_int32]),
-    "jfm_mlp_forward": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_void_p]),
+    "jfm_mlp_forward": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "jfm_mlp_backward": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "jfm_mlp_adam": (c_int32, [c_void_p, c_int32, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),

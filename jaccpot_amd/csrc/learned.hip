@@ -200,7 +200,11 @@ __device__ __forceinline__ void mlp_forward_one(const float* w, const float* x, 
   }
 }
 
+// mean / inv_std (optional, [kIn]): features are normalised on the fly,
+// x_j = (X_j - mean_j) * inv_std_j.
 __global__ __launch_bounds__(kThreads) void mlp_forward_kernel(int32_t K, const float* __restrict__ X,
+                                                               const float* __restrict__ mean,
+                                                               const float* __restrict__ inv_std,
                                                                const uint16_t* __restrict__ wbf,
                                                                float* __restrict__ Y) {
   __shared__ float w[kParams];
@@ -210,6 +214,8 @@ __global__ __launch_bounds__(kThreads) void mlp_forward_kernel(int32_t K, const 
   if (k >= K) return;
   float x[kIn], h1[kHid], h2[kHid], y[kOut];
   for (int j = 0; j < kIn; ++j) x[j] = X[static_cast<int64_t>(k) * kIn + j];
+  if (mean)
+    for (int j = 0; j < kIn; ++j) x[j] = (x[j] - mean[j]) * inv_std[j];
   mlp_forward_one(w, x, h1, h2, y);
   for (int j = 0; j < kOut; ++j) Y[static_cast<int64_t>(k) * kOut + j] = y[j];
 }
@@ -438,11 +444,13 @@ int32_t jfm_pair_labels(void* stream, int32_t K, const int32_t* tgt_of, const in
   return check_launch("jfm_pair_labels");
 }
 
-int32_t jfm_mlp_forward(void* stream, int32_t K, const float* X, const uint16_t* wbf16, float* Y) {
+int32_t jfm_mlp_forward(void* stream, int32_t K, const float* X, const float* mean, const float* inv_std,
+                        const uint16_t* wbf16, float* Y) {
   if (K < 0) return fail(JFM_ERR_ARG, "jfm_mlp_forward: bad sizes");
   if (K == 0) return JFM_OK;
   if (!X || !wbf16 || !Y) return fail(JFM_ERR_ARG, "jfm_mlp_forward: NULL buffer");
-  mlp_forward_kernel<<<nblocks(K, kThreads), kThreads, 0, as_stream(stream)>>>(K, X, wbf16, Y);
+  if ((mean == nullptr) != (inv_std == nullptr)) return fail(JFM_ERR_ARG, "jfm_mlp_forward: mean without inv_std");
+  mlp_forward_kernel<<<nblocks(K, kThreads), kThreads, 0, as_stream(stream)>>>(K, X, mean, inv_std, wbf16, Y);
   return check_launch("jfm_mlp_forward");
 }
 
