@@ -282,6 +282,10 @@ int32_t jfm_mlp_num_params(void);
 int64_t jfm_mlp_backward_scratch_floats(int32_t K, int32_t per_block);
 int32_t jfm_mlp_forward(void* stream, int32_t K, const float* X, const float* mean, const float* inv_std,
                         const uint16_t* wbf16, float* Y);
+/* The same forward pass on the matrix cores (bf16 MFMA, fp32 accumulation;
+ * activations rounded to bf16 between layers): the inference path. */
+int32_t jfm_mlp_forward_mfma(void* stream, int32_t K, const float* X, const float* mean, const float* inv_std,
+                             const uint16_t* wbf16, float* Y);
 int32_t jfm_mlp_backward(void* stream, int32_t K, int32_t per_block, const float* X, const float* targets,
                          const uint16_t* wbf16, float* scratch, float* grad_and_loss);
 int32_t jfm_mlp_adam(void* stream, int32_t step, float lr, float beta1, float beta2, float eps, float* master,

@@ -85,6 +85,7 @@ _SIGNATURES = {
     "jfm_mlp_num_params": (c_int32, []),
     "jfm_mlp_backward_scratch_floats": (c_int64, [c_int32, c_int32]),
     "jfm_mlp_forward": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "jfm_mlp_forward_mfma": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "jfm_mlp_backward": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "jfm_mlp_adam": (c_int32, [c_void_p, c_int32, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),

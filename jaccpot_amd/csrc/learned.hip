@@ -220,6 +220,108 @@ __global__ __launch_bounds__(kThreads) void mlp_forward_kernel(int32_t K, const 
   for (int j = 0; j < kOut; ++j) Y[static_cast<int64_t>(k) * kOut + j] = y[j];
 }
 
+// Forward on the matrix cores (bf16 MFMA, fp32 accumulation), 32 pairs per
+// wave tile, layers computed transposed: H1^T = relu(W1 X^T + b1),
+// H2^T = relu(W2 H1^T + b2), Y^T = W3 H2^T + b3 with v_mfma_f32_32x32x8bf16.
+// Operand layout of that instruction (64 lanes, h = lane / 32):
+//   A[i][k]: lane i + 32 h holds k = 4h .. 4h+3;  B[k][j]: lane j + 32 h holds
+//   k = 4h .. 4h+3;  C[i][j]: lane j + 32 h, element v holds i = 8(v/4) + 4h + v%4,
+// so the 16 outputs a lane holds for pair j are exactly the k-rows the next
+// layer's B operand needs from it (rows 8r + 4h .. +3 for k-block r): the
+// activations never leave registers.  Activations are rounded to bf16 for the
+// next product (the weights are bf16 in every path).
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ bf16x4_t pack_bf16(float a, float b, float c, float d) {
+  bf16x4_t r;
+  r[0] = static_cast<short>(f32_to_bf16(a));
+  r[1] = static_cast<short>(f32_to_bf16(b));
+  r[2] = static_cast<short>(f32_to_bf16(c));
+  r[3] = static_cast<short>(f32_to_bf16(d));
+  return r;
+}
+
+// 4 consecutive bf16 weights W[row][col .. col+3] of a row-major block, or
+// zeros for rows >= nrows (padding of the 3-output layer).
+__device__ __forceinline__ bf16x4_t weight_frag(const uint16_t* __restrict__ wbf, int off, int ncols, int row,
+                                                int col, int nrows) {
+  bf16x4_t r = {0, 0, 0, 0};
+  if (row < nrows)
+    for (int q = 0; q < 4; ++q) r[q] = static_cast<short>(wbf[off + row * ncols + col + q]);
+  return r;
+}
+
+__global__ __launch_bounds__(kThreads) void mlp_forward_mfma_kernel(int32_t K, const float* __restrict__ X,
+                                                                    const float* __restrict__ mean,
+                                                                    const float* __restrict__ inv_std,
+                                                                    const uint16_t* __restrict__ wbf,
+                                                                    float* __restrict__ Y) {
+  constexpr int kWave = 64;
+  const int lane = threadIdx.x % kWave;
+  const int h = lane / 32;
+  const int c32 = lane % 32;
+  // weight fragments (A operands), loaded once per wave
+  bf16x4_t a1[2], a2[2][8], a3[8];
+  for (int ob = 0; ob < 2; ++ob) a1[ob] = weight_frag(wbf, oW1, kIn, ob * 32 + c32, 4 * h, kHid);
+  for (int ob = 0; ob < 2; ++ob)
+    for (int kb = 0; kb < 8; ++kb) a2[ob][kb] = weight_frag(wbf, oW2, kHid, ob * 32 + c32, kb * 8 + 4 * h, kHid);
+  for (int kb = 0; kb < 8; ++kb) a3[kb] = weight_frag(wbf, oW3, kHid, c32, kb * 8 + 4 * h, kOut);
+  // biases in the C layout: element v of block ob is row ob*32 + 8(v/4) + 4h + v%4
+  float bias1[2][16], bias2[2][16], bias3[16];
+  for (int v = 0; v < 16; ++v) {
+    const int r = 8 * (v / 4) + 4 * h + (v % 4);
+    for (int ob = 0; ob < 2; ++ob) {
+      bias1[ob][v] = bf16_to_f32(wbf[oB1 + ob * 32 + r]);
+      bias2[ob][v] = bf16_to_f32(wbf[oB2 + ob * 32 + r]);
+    }
+    bias3[v] = r < kOut ? bf16_to_f32(wbf[oB3 + r]) : 0.0f;
+  }
+  float mu[4] = {0, 0, 0, 0}, is[4] = {1, 1, 1, 1};
+  if (mean)
+    for (int q = 0; q < 4; ++q) {
+      mu[q] = mean[4 * h + q];
+      is[q] = inv_std[4 * h + q];
+    }
+
+  const int tiles = (K + 31) / 32;
+  const int waves = gridDim.x * (kThreads / kWave);
+  for (int t = blockIdx.x * (kThreads / kWave) + static_cast<int>(threadIdx.x) / kWave; t < tiles; t += waves) {
+    const int pair = t * 32 + c32;
+    // B operand of layer 1: features 4h .. 4h+3 of this lane's pair
+    float x[4] = {0, 0, 0, 0};
+    if (pair < K)
+      for (int q = 0; q < 4; ++q) x[q] = (X[static_cast<int64_t>(pair) * kIn + 4 * h + q] - mu[q]) * is[q];
+    const bf16x4_t bx = pack_bf16(x[0], x[1], x[2], x[3]);
+    // layer 1
+    bf16x4_t b2op[8];
+    for (int ob = 0; ob < 2; ++ob) {
+      f32x16_t c;
+      for (int v = 0; v < 16; ++v) c[v] = bias1[ob][v];
+      c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a1[ob], bx, c, 0, 0, 0);
+      for (int r = 0; r < 4; ++r)
+        b2op[ob * 4 + r] = pack_bf16(fmaxf(c[4 * r], 0.0f), fmaxf(c[4 * r + 1], 0.0f), fmaxf(c[4 * r + 2], 0.0f),
+                                     fmaxf(c[4 * r + 3], 0.0f));
+    }
+    // layer 2
+    bf16x4_t b3op[8];
+    for (int ob = 0; ob < 2; ++ob) {
+      f32x16_t c;
+      for (int v = 0; v < 16; ++v) c[v] = bias2[ob][v];
+      for (int kb = 0; kb < 8; ++kb) c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a2[ob][kb], b2op[kb], c, 0, 0, 0);
+      for (int r = 0; r < 4; ++r)
+        b3op[ob * 4 + r] = pack_bf16(fmaxf(c[4 * r], 0.0f), fmaxf(c[4 * r + 1], 0.0f), fmaxf(c[4 * r + 2], 0.0f),
+                                     fmaxf(c[4 * r + 3], 0.0f));
+    }
+    // layer 3 (rows 0..2 of the output block; rows 3..31 are padding)
+    f32x16_t c;
+    for (int v = 0; v < 16; ++v) c[v] = bias3[v];
+    for (int kb = 0; kb < 8; ++kb) c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a3[kb], b3op[kb], c, 0, 0, 0);
+    if (h == 0 && pair < K)
+      for (int v = 0; v < kOut; ++v) Y[static_cast<int64_t>(pair) * kOut + v] = c[v];
+  }
+}
+
 // Per-workgroup partial gradient of loss = mean_k mean_j (y_kj - t_kj)^2 over
 // pairs [blockIdx.x * per_block, ...).  partial[block][kParams + 1] (last = loss sum).
 __global__ __launch_bounds__(kThreads) void mlp_backward_kernel(int32_t K, int32_t per_block,
@@ -452,6 +554,18 @@ int32_t jfm_mlp_forward(void* stream, int32_t K, const float* X, const float* me
   if ((mean == nullptr) != (inv_std == nullptr)) return fail(JFM_ERR_ARG, "jfm_mlp_forward: mean without inv_std");
   mlp_forward_kernel<<<nblocks(K, kThreads), kThreads, 0, as_stream(stream)>>>(K, X, mean, inv_std, wbf16, Y);
   return check_launch("jfm_mlp_forward");
+}
+
+int32_t jfm_mlp_forward_mfma(void* stream, int32_t K, const float* X, const float* mean, const float* inv_std,
+                             const uint16_t* wbf16, float* Y) {
+  if (K < 0) return fail(JFM_ERR_ARG, "jfm_mlp_forward_mfma: bad sizes");
+  if (K == 0) return JFM_OK;
+  if (!X || !wbf16 || !Y) return fail(JFM_ERR_ARG, "jfm_mlp_forward_mfma: NULL buffer");
+  if ((mean == nullptr) != (inv_std == nullptr)) return fail(JFM_ERR_ARG, "jfm_mlp_forward_mfma: mean without inv_std");
+  const int tiles = (K + 31) / 32;
+  const int blocks = min((tiles + 3) / 4, 256 * 8);  // 4 waves per workgroup, tiles grid-strided
+  mlp_forward_mfma_kernel<<<blocks, kThreads, 0, as_stream(stream)>>>(K, X, mean, inv_std, wbf16, Y);
+  return check_launch("jfm_mlp_forward_mfma");
 }
 
 int32_t jfm_mlp_backward(void* stream, int32_t K, int32_t per_block, const float* X, const float* targets,

@@ -82,7 +82,7 @@ def test_mlp_kernels_match_oracle(require_gpu):
     T = rng.normal(size=(K, 3)).astype(np.float32)
     Xd = torch.as_tensor(X, device="cuda")
     Td = torch.as_tensor(T, device="cuda")
-    Y = model.predict(Xd, normalised=True).cpu().numpy()
+    Y = model.predict(Xd, normalised=True, exact=True).cpu().numpy()
     theta = omlp.bf16_round(model.master.cpu().numpy())
     Yr, _ = omlp.forward(theta, X)
     assert np.abs(Y - Yr).max() <= 2e-5 * max(1.0, np.abs(Yr).max())
@@ -99,6 +99,43 @@ def test_mlp_kernels_match_oracle(require_gpu):
     model.adam(grad, lr=1e-3)
     step = (model.master - before).cpu().numpy()
     assert np.all(step * g <= 1e-12)
+
+
+def test_mfma_forward_matches_oracle(require_gpu):
+    """The matrix-core forward (bf16 activations between layers) against the
+    float64 oracle with the same bf16 weights, on random and on real features."""
+    from jaccpot_amd import _native as nat
+    from jaccpot_amd.learned import OrderModel
+
+    print("[gpu] mlp forward on MFMA", flush=True)
+    model = OrderModel(seed=9)
+    rng = np.random.default_rng(4)
+    for K in (1, 31, 32, 33, 4097, 70001):
+        X = rng.normal(size=(K, 8)).astype(np.float32)
+        Xd = torch.as_tensor(X, device="cuda")
+        Y = torch.empty((K, 3), dtype=torch.float32, device="cuda")
+        nat.call("jfm_mlp_forward_mfma", nat.current_stream(), K, Xd.data_ptr(), None, None, model.wbf16.data_ptr(),
+                 Y.data_ptr())
+        theta = omlp.bf16_round(model.master.cpu().numpy())
+        Yr, _ = omlp.forward(theta, X)
+        err = float(np.abs(Y.cpu().numpy() - Yr).max())
+        scale = max(1.0, float(np.abs(Yr).max()))
+        print(f"[gpu]   K={K}: max |Y - Y_oracle| {err:.3e} (scale {scale:.2f})", flush=True)
+        assert err <= 3e-2 * scale
+    # in-kernel normalisation: same result as normalising on the host first
+    mean = rng.normal(size=8).astype(np.float32)
+    std = rng.uniform(0.5, 2.0, size=8).astype(np.float32)
+    model._set_stats(mean, std)
+    X = rng.normal(size=(5000, 8)).astype(np.float32) * std + mean
+    Xd = torch.as_tensor(X, device="cuda")
+    Y1 = torch.empty((5000, 3), dtype=torch.float32, device="cuda")
+    Y2 = torch.empty_like(Y1)
+    nat.call("jfm_mlp_forward_mfma", nat.current_stream(), 5000, Xd.data_ptr(), model.feature_mean.data_ptr(),
+             model.feature_inv_std.data_ptr(), model.wbf16.data_ptr(), Y1.data_ptr())
+    Xn = torch.as_tensor(((X - mean) * (1.0 / std)).astype(np.float32), device="cuda")
+    nat.call("jfm_mlp_forward_mfma", nat.current_stream(), 5000, Xn.data_ptr(), None, None, model.wbf16.data_ptr(),
+             Y2.data_ptr())
+    assert torch.allclose(Y1, Y2, atol=1e-5, rtol=0)
 
 
 def _train_model(seed=0):
