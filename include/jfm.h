@@ -256,8 +256,9 @@ int32_t jfm_leaf_adjoint_b(void* stream, int32_t dtype, int32_t p, const void* p
  * Features [K][8] of far pairs (k -> target tgt_of[k], source far_src[k]):
  * log2(d/(r_t+r_s)), log10(m_s/m_total), log10(m_t/m_total), and
  * log10(|M_s,n|/(m_s r_s^n)) for n = 2..6; radius [nodes] float32 bounding radii.
- * Labels [K][3]: log10 of the r_t^n-weighted relative truncation error of the
- * order 2/4/6 translation against order 8 (multipoles8 at order 8, float32).
+ * Labels [K][3]: log10 of the relative force error on the target sphere of the
+ * order gears3[j] translation against order 8 (degree weights n r_t^(n-1),
+ * rotation-invariant degree norm; floor 1e-7; multipoles8 at order 8, float32).
  * MLP 8 -> 64 -> 64 -> 3 (ReLU), jfm_mlp_num_params() float32 parameters
  * W1[64][8] b1 W2[64][64] b2 W3[3][64] b3, used as bfloat16 (wbf16) with
  * float32 accumulation.  jfm_mlp_backward: gradient of the mean squared error
