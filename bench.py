@@ -63,16 +63,21 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline_sample(gen: str, p: int, leaf: int, theta: float, eps: float, n_sample: int):
-    """The float64 oracle FMM (NumPy restatement, single thread) on a bounded sample."""
-    from oracle import fmm as ofmm
-    from oracle import ics as oics
+def cpu_baseline(x, m, p: int, leaf: int, theta: float, eps: float):
+    """The plain fixed-order FMM timed on the host: oracle/cfmm.cpp (the C++/OpenMP
+    restatement of the reference's pipeline), float32 arithmetic, every OpenMP
+    thread, one evaluation (gather, P2M, M2M, M2L, L2L, L2P + P2P) on the native
+    host topology, which is built first and reported separately (as on the GPU)."""
+    from jaccpot_amd.topology import build_topology
+    from oracle import cfmm
 
-    x, m = getattr(oics, gen)(n_sample, 12)
+    topo = build_topology(x, m, leaf_size=leaf, theta=theta)
+    threads = int(cfmm.load().cfmm_max_threads())
+    stages = {}
     t0 = time.perf_counter()
-    ofmm.fmm_accelerations(x, m, leaf_size=leaf, p=p, theta=theta, eps=eps)
+    cfmm.evaluate(topo, x, m, p=p, eps=eps, precision="fp32", timings=stages)
     dt = time.perf_counter() - t0
-    return n_sample / dt, dt
+    return x.shape[0] / dt, dt, threads, stages, topo.build_seconds
 
 
 def main() -> int:
@@ -88,7 +93,8 @@ def main() -> int:
                     help="also time the step replayed from a captured HIP graph and report that as ms_per_step")
     ap.add_argument("--no-grad", action="store_true", help="skip the differentiable-path timing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=16384)
+    ap.add_argument("--cpu-sample", type=int, default=0,
+                    help="particles of the CPU baseline run (0: the full workload)")
     ap.add_argument("--error-targets", type=int, default=4096)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="run one torch copy of this many MiB first (HBM counter calibration)")
@@ -276,13 +282,18 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu_baseline:
-        cn = min(args.cpu_sample, N)
-        rate, dt = cpu_baseline_sample(gen, p, leaf, theta, eps, cn)
-        cpu = {"value": rate, "unit": "particles/s", "cores": 1, "kind": "port",
-               "sample": f"float64 NumPy oracle FMM, full pipeline (tree, lists, P2M..P2P), {gen} N={cn}, "
-                         f"p={p}, leaf={leaf}, theta={theta}, one process ({dt:.1f}s)",
+        cn = N if args.cpu_sample <= 0 else min(args.cpu_sample, N)
+        xc, mc = (x, m) if cn == N else getattr(ics, gen)(cn, seed)
+        rate, dt, threads, stages, topo_s = cpu_baseline(xc, mc, p, leaf, theta, eps)
+        cpu = {"value": rate, "unit": "particles/s", "cores": threads, "kind": "port",
+               "sample": f"C++/OpenMP restatement of the plain fixed-order FMM (oracle/cfmm.cpp), float32, "
+                         f"{gen} N={cn} p={p} leaf={leaf} theta={theta} softening={eps}: one evaluation "
+                         f"(gather, P2M, M2M, M2L, L2L, L2P+P2P) in {dt:.1f}s on {threads} threads; "
+                         f"host topology build {topo_s:.1f}s not included",
+               "stage_s": {k: round(v, 3) for k, v in stages.items()},
                "cpu": platform_cpu()}
-        log(f"cpu baseline {rate:.3e} particles/s on N={cn} ({dt:.1f}s)")
+        log(f"cpu baseline {rate:.3e} particles/s on N={cn} ({dt:.1f}s, {threads} threads) "
+            f"stages {json.dumps({k: round(v, 2) for k, v in stages.items()})}")
 
     value = N / (ms_per_step * 1e-3)
     if learned is not None:

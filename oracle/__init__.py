@@ -16,6 +16,11 @@ topology   -- the fixed static-radix topology spec (Morton sort, leaf buckets,
               binary radix tree, COM geometry, dual-tree far/near lists).
 fmm        -- the full fixed-order FMM (P2M -> M2M -> M2L -> L2L -> L2P + P2P).
 direct     -- the float64 O(N^2) direct-sum oracle.
+cfmm       -- ctypes wrapper of cfmm.cpp, a C++/OpenMP restatement of the
+              fmm numerics (float64 or float32, optional per-pair orders) on a
+              given topology: the checker at large N and the CPU baseline.
+labels     -- per-pair truncation-error labels of the learned order.
+mlp        -- the order network's forward/backward in NumPy.
 ics        -- seeded initial-condition generators for the benchmark configs.
 
 Parity status of each function is recorded in its docstring ("pinned by ...").

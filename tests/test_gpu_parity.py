@@ -281,3 +281,68 @@ def test_graph_replay_matches_eager(learned, require_gpu):
         fmm._evaluate_device(state, acc_e, pot_e)
         torch.cuda.synchronize()
         assert torch.equal(acc_g, acc_e) and torch.equal(pot_g, pot_e), k
+
+
+def _cfmm():
+    from oracle import cfmm
+
+    if not cfmm.LIBRARY.exists():
+        cfmm.build()
+    return cfmm
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_262k_plummer_matches_cpu_restatement(precision, require_gpu):
+    """Plummer N=262,144 (p 4, leaf 32, theta 0.6, softening 1e-3): the GPU FMM
+    against the C++ restatement in float64 on the same topology (accelerations
+    and potentials)."""
+    from jaccpot_amd import FastMultipoleMethod
+
+    cfmm = _cfmm()
+    print(f"[gpu] 262k plummer vs C++ restatement {precision}", flush=True)
+    x, m = ics.plummer(262144, 7)
+    if precision == "fp32":
+        x, m = x.astype(np.float32), m.astype(np.float32)
+    fmm = FastMultipoleMethod(precision=precision, theta=0.6, softening=1e-3)
+    state = fmm.prepare_state(x, m, leaf_size=32, max_order=4)
+    acc, pot = fmm.evaluate_prepared_state(state, return_potential=True)
+    torch.cuda.synchronize()
+    ref, rpot = cfmm.evaluate(state.topology, x.astype(np.float64), m.astype(np.float64), p=4, eps=1e-3,
+                              return_potential=True)
+    ra = np.linalg.norm(acc - ref) / np.linalg.norm(ref)
+    rp = np.linalg.norm(pot - rpot) / np.linalg.norm(rpot)
+    print(f"[gpu]   rel-L2 acc {ra:.3e} pot {rp:.3e}", flush=True)
+    tol = 1e-12 if precision == "fp64" else 1e-5
+    assert ra < tol and rp < tol
+
+
+def test_benchmark_workload_matches_cpu_restatement(require_gpu):
+    """The benchmark's learned+mixed configuration at full size -- Plummer
+    N=4,194,304 (seed 2), float32, leaf 8, theta 0.85, per-pair gears (3, 6, 8)
+    from the shipped network at tolerance -1.5, float64 top levels -- against
+    the C++ restatement in float64 on the same topology with the same per-pair
+    orders: relative L2 difference of the accelerations below 1e-5."""
+    from jaccpot_amd import FastMultipoleMethod
+    from jaccpot_amd.train_order import load, weights_path
+
+    cfmm = _cfmm()
+    print("[gpu] benchmark workload (4M learned+mixed) vs C++ restatement", flush=True)
+    x, m = ics.plummer(4_194_304, 2)
+    x32, m32 = x.astype(np.float32), m.astype(np.float32)
+    fmm = FastMultipoleMethod(precision="fp32", theta=0.85, softening=1e-3, mixed_precision=True,
+                              adaptive_order=True, p_gears=(3, 6, 8), order_model=load(weights_path((3, 6, 8))),
+                              gear_tolerance=-1.5)
+    state = fmm.prepare_state(x32, m32, leaf_size=8, max_order=8)
+    acc = fmm.evaluate_prepared_state(state)
+    torch.cuda.synchronize()
+    orders = state.extra["gear_of_pair"]
+    counts = {int(g): int(np.sum(orders == g)) for g in (3, 6, 8)}
+    print(f"[gpu]   GPU done; far pairs {orders.size}, gears {counts}; running the C++ restatement", flush=True)
+    assert sum(counts.values()) == orders.size and min(counts.values()) > 0
+    ref = cfmm.evaluate(state.topology, x32.astype(np.float64), m32.astype(np.float64), p=8, eps=1e-3,
+                        orders=orders)
+    rel = np.linalg.norm(acc - ref) / np.linalg.norm(ref)
+    per = np.linalg.norm(acc - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    print(f"[gpu]   rel-L2 {rel:.3e}; per particle median {np.median(per):.3e} p99 {np.quantile(per, 0.99):.3e}",
+          flush=True)
+    assert rel < 1e-5
