@@ -98,6 +98,65 @@ int32_t jfm_topology_schedules(const jfm_topology* topo, int32_t* m2m_nodes, int
 
 void jfm_topology_free(jfm_topology* topo);
 
+/* ------------------------------------------------------------------------
+ * Device topology (the same yggdrax calls as above, jaccpot/runtime/
+ * fmm_state.py:630-800 and runtime/_interaction_cache.py:523,564, run on the
+ * GPU): the steps of jfm_topology_build as stream-ordered kernels whose
+ * outputs equal the host build's bit for bit.  Positions [n][3] and masses [n]
+ * are float64 device arrays in input order.  The caller sequences the steps
+ * (jaccpot_amd/dtopology.py), reading back two small counts in between.
+ * ---------------------------------------------------------------------- */
+/* Bounds: bounds6 = {lo, hi} by exact min/max unless given != 0 (then bounds6
+ * already holds them); flags |= 1 if any coordinate is not finite.  scratch
+ * holds jfm_tree_bounds_scratch_doubles() doubles. */
+int32_t jfm_tree_bounds_scratch_doubles(void);
+int32_t jfm_tree_bounds(void* stream, const double* pos, int64_t n, int32_t given, double* scratch,
+                        double* bounds6, int32_t* flags);
+/* 63-bit Morton codes on the cube of bounds6 and index[i] = i. */
+int32_t jfm_tree_morton(void* stream, const double* pos, int64_t n, const double* bounds6, uint64_t* codes,
+                        int32_t* index);
+/* Stable LSD radix sort of uint64 keys over bits [0, end_bit), optionally
+ * carrying int32 values (both value pointers NULL: keys only); scratch holds
+ * jfm_sort_scratch_bytes(n, with_values) bytes. */
+int32_t jfm_sort_scratch_bytes(int64_t n, int32_t with_values, int64_t* bytes);
+int32_t jfm_sort_u64(void* stream, int64_t n, int32_t end_bit, const uint64_t* keys_in, uint64_t* keys_out,
+                     const int32_t* vals_in, int32_t* vals_out, void* scratch, int64_t scratch_bytes);
+/* Binary radix tree over the leaf keys sorted_codes[l * leaf_size]: parent
+ * [nodes] (-1 at the root), left/right [L-1], node_range [nodes][2]. */
+int32_t jfm_tree_radix(void* stream, int64_t n, int32_t leaf_size, const uint64_t* sorted_codes, int32_t* parent,
+                       int32_t* left, int32_t* right, int32_t* node_range);
+/* depth [nodes] by the parent chain, *max_depth (zeroed by the caller)
+ * raised to the largest; flags |= 2 if a chain is longer than any radix tree
+ * can be.  Level keys for the schedules: m2m_keys [I] = (D - depth) << 32 | node,
+ * l2l_keys [nodes-1] = (depth - 1) << 32 | node for nodes 1..nodes-1. */
+int32_t jfm_tree_levels(void* stream, int32_t nodes, int32_t I, const int32_t* parent, int32_t* depth,
+                        int32_t* max_depth, int32_t* flags, uint64_t* m2m_keys, uint64_t* l2l_keys);
+/* Centres of mass: every leaf (slot order sums), then one call per internal
+ * level, deepest first (level_nodes = that level's slice of the M2M schedule). */
+int32_t jfm_tree_leaf_geometry(void* stream, int32_t L, int32_t I, const double* pos, const double* mass,
+                               const int32_t* perm, const int32_t* node_range, double* node_mass, double* center);
+int32_t jfm_tree_level_geometry(void* stream, int32_t count, const int32_t* level_nodes, const double* pos,
+                                const double* mass, const int32_t* perm, const int32_t* left, const int32_t* right,
+                                const int32_t* node_range, double* node_mass, double* center);
+/* radius [nodes] = max distance from the centre to the node's particles. */
+int32_t jfm_tree_radius(void* stream, int32_t nodes, const double* pos, const int32_t* perm,
+                        const int32_t* node_range, const double* center, double* radius);
+/* One breadth-first step of the dual-tree walk: expands items_in [n_in][2]
+ * into items_out, appending far pairs (target << 32 | source node) and near
+ * pairs (target << 32 | source leaf) after counts[1] / counts[2] entries;
+ * counts[0] (zeroed by the caller) receives the new item count.  A count
+ * above its capacity means that output was truncated: re-run the step with
+ * larger buffers and counts reset. */
+int32_t jfm_tree_walk_step(void* stream, int32_t I, const int32_t* items_in, int64_t n_in, const int32_t* left,
+                           const int32_t* right, const double* center, const double* radius, double theta,
+                           int32_t* items_out, int64_t cap_items, uint64_t* far_keys, int64_t cap_far,
+                           uint64_t* near_keys, int64_t cap_near, uint64_t* counts);
+/* Sorted (row << 32 | col) keys -> CSR off [rows+1], col [K]. */
+int32_t jfm_keys_to_csr(void* stream, const uint64_t* sorted_keys, int64_t K, int32_t rows, int32_t* off,
+                        int32_t* col);
+/* Number of significant bits of max_key (sort widths). */
+int32_t jfm_key_bits(uint64_t max_key);
+
 /* Dehnen real multipole swap matrices B_U for degrees 0..p, float64, packed
  * row-major block after block (sum (2l+1)^2 entries); the constants every
  * rotation kernel is compiled with (jaccpot/operators/real_dehnen_q.py:343-470). */

@@ -25,7 +25,7 @@ EVAL_FAR = 2
 EVAL_POTENTIAL = 4
 EVAL_COMPENSATED = 8
 MAX_ORDER = 8
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 _i32p = POINTER(c_int32)
 _f64p = POINTER(c_double)
@@ -42,6 +42,22 @@ _SIGNATURES = {
     "jfm_topology_export": (c_int32, [c_void_p] + [c_void_p] * 13),
     "jfm_topology_schedules": (c_int32, [c_void_p] + [c_void_p] * 5),
     "jfm_topology_free": (None, [c_void_p]),
+    "jfm_tree_bounds_scratch_doubles": (c_int32, []),
+    "jfm_tree_bounds": (c_int32, [c_void_p, c_void_p, c_int64, c_int32, c_void_p, c_void_p, c_void_p]),
+    "jfm_tree_morton": (c_int32, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p]),
+    "jfm_sort_scratch_bytes": (c_int32, [c_int64, c_int32, POINTER(c_int64)]),
+    "jfm_sort_u64": (c_int32, [c_void_p, c_int64, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                               c_int64]),
+    "jfm_tree_radix": (c_int32, [c_void_p, c_int64, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "jfm_tree_levels": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                  c_void_p]),
+    "jfm_tree_leaf_geometry": (c_int32, [c_void_p, c_int32, c_int32] + [c_void_p] * 6),
+    "jfm_tree_level_geometry": (c_int32, [c_void_p, c_int32] + [c_void_p] * 9),
+    "jfm_tree_radius": (c_int32, [c_void_p, c_int32] + [c_void_p] * 5),
+    "jfm_tree_walk_step": (c_int32, [c_void_p, c_int32, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
+                                     c_double, c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_int64, c_void_p]),
+    "jfm_keys_to_csr": (c_int32, [c_void_p, c_void_p, c_int64, c_int32, c_void_p, c_void_p]),
+    "jfm_key_bits": (c_int32, [ctypes.c_uint64]),
     "jfm_tables_bu": (c_int32, [c_int32, c_void_p, c_int64]),
     "jfm_gather_sorted": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_int32, c_void_p]),
     "jfm_p2m": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_int32, c_int32, c_int32, c_int32, c_void_p,
@@ -135,7 +151,8 @@ def call(name: str, *args) -> int:
     """Call an entry point and raise JfmError on a non-zero status."""
     rc = getattr(load(), name)(*args)
     if name not in ("jfm_direct_scratch_doubles", "jfm_coeff_stride", "jfm_order_supported", "jfm_abi_version",
-                    "jfm_mlp_num_params", "jfm_mlp_backward_scratch_floats"):
+                    "jfm_mlp_num_params", "jfm_mlp_backward_scratch_floats", "jfm_tree_bounds_scratch_doubles",
+                    "jfm_key_bits"):
         check(rc, name)
     return rc
 
