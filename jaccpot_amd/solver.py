@@ -26,6 +26,7 @@ import torch
 
 from . import _native as nat
 from .flatlist import build_flat_list, launch_flat, prefers_flat
+from .dtopology import build_topology_device
 from .topology import Topology, build_topology
 
 _PRESETS = ("fast", "balanced", "accurate", "large_n_gpu")
@@ -98,7 +99,9 @@ class FastMultipoleMethod:
     truncation error times m_s / d^2) is at most ``10**gear_tolerance`` times
     the largest such error of the top gear among the target's pairs (the
     largest gear otherwise); ``max_order`` must be the largest gear.  Gears
-    are fixed in ``prepare_state``.
+    are fixed in ``prepare_state``.  ``tree_build`` picks where ``prepare_state``
+    builds the topology: ``"device"`` (HIP kernels, default) or ``"host"``
+    (C++ threads); both give the same arrays bit for bit.
     """
 
     def __init__(
@@ -119,6 +122,7 @@ class FastMultipoleMethod:
         order_model: Any = None,
         gear_tolerance: float = -2.5,
         num_threads: int = 0,
+        tree_build: str = "device",
         **legacy_kwargs: Any,
     ):
         preset_norm = str(getattr(preset, "value", preset)).strip().lower()
@@ -142,6 +146,8 @@ class FastMultipoleMethod:
             raise ValueError("theta must be > 0")
         if not (softening >= 0.0):
             raise ValueError("softening must be >= 0")
+        if tree_build not in ("device", "host"):
+            raise ValueError("tree_build must be 'device' or 'host'")
         dtype = None
         if precision is not None:
             p = str(precision).strip().lower()
@@ -167,6 +173,7 @@ class FastMultipoleMethod:
         self.order_model = order_model
         self.gear_tolerance = float(gear_tolerance)
         self.num_threads = int(num_threads)
+        self.tree_build = tree_build
         self.device = _require_gpu()
         self._last_state: Optional[FMMPreparedState] = None
 
@@ -211,29 +218,38 @@ class FastMultipoleMethod:
         x_host, m_host, kind, in_dtype, in_dev = self._as_inputs(positions, masses)
         if not np.all(np.isfinite(x_host)) or not np.all(np.isfinite(m_host)):
             raise ValueError("positions and masses must be finite")
-        topo = build_topology(x_host, m_host, leaf_size=int(leaf_size), theta=th, bounds=bounds,
-                              num_threads=self.num_threads)
         dtype = self._work_dtype(in_dtype)
         dev = self.device
+        xd = torch.as_tensor(x_host, device=dev)
+        md = torch.as_tensor(m_host, device=dev)
+        if self.tree_build == "device":
+            topo = build_topology_device(xd, md, leaf_size=int(leaf_size), theta=th, bounds=bounds)
+        else:
+            topo = build_topology(x_host, m_host, leaf_size=int(leaf_size), theta=th, bounds=bounds,
+                                  num_threads=self.num_threads)
+        on_dev = topo.extra.get("device", {})
         code = nat.dtype_code(dtype)
         cs = nat.coeff_stride(p, dtype)
 
-        def i32(a):
-            return torch.as_tensor(np.ascontiguousarray(a, np.int32)).to(dev)
+        def i32(a, name=None):
+            d = on_dev.get(name) if name else None
+            if d is not None and d.numel() > 0:
+                return d
+            a = np.asarray(a)
+            return torch.as_tensor(np.ascontiguousarray(a if a.size else np.zeros(1), np.int32)).to(dev)
 
         nodes = topo.num_nodes
         state = FMMPreparedState(
             topology=topo, order=p, dtype=dtype, G=self.G, softening=self.softening, theta=th, device=dev,
             input_kind=kind, input_dtype=in_dtype, input_device=in_dev, cs=cs,
-            perm=i32(topo.perm),
+            perm=i32(topo.perm, "perm"),
             pos4=torch.empty((topo.num_particles, 4), dtype=dtype, device=dev),
-            node_range=i32(topo.node_range), left=i32(topo.left if topo.left.size else np.zeros(1)),
-            right=i32(topo.right if topo.right.size else np.zeros(1)), parent=i32(topo.parent),
-            m2m_nodes=i32(topo.m2m_nodes if topo.m2m_nodes.size else np.zeros(1)),
-            l2l_nodes=i32(topo.l2l_nodes if topo.l2l_nodes.size else np.zeros(1)),
-            m2l_targets=i32(topo.m2l_targets if topo.m2l_targets.size else np.zeros(1)),
-            far_off=i32(topo.far_off), far_src=i32(topo.far_src if topo.far_src.size else np.zeros(1)),
-            near_off=i32(topo.near_off), near_src=i32(topo.near_src if topo.near_src.size else np.zeros(1)),
+            node_range=i32(topo.node_range, "node_range"), left=i32(topo.left, "left"),
+            right=i32(topo.right, "right"), parent=i32(topo.parent, "parent"),
+            m2m_nodes=i32(topo.m2m_nodes, "m2m_nodes"), l2l_nodes=i32(topo.l2l_nodes, "l2l_nodes"),
+            m2l_targets=i32(topo.m2l_targets),
+            far_off=i32(topo.far_off, "far_off"), far_src=i32(topo.far_src, "far_src"),
+            near_off=i32(topo.near_off, "near_off"), near_src=i32(topo.near_src, "near_src"),
             centers4=torch.zeros((nodes, 4), dtype=dtype, device=dev),
             multipoles=torch.zeros((nodes, cs), dtype=dtype, device=dev),
             locals=torch.zeros((nodes, cs), dtype=dtype, device=dev),
@@ -244,8 +260,6 @@ class FastMultipoleMethod:
         if prefers_flat(topo.num_far_pairs, int(topo.m2l_targets.size)):
             state.extra["flat"] = build_flat_list(np.repeat(np.arange(nodes, dtype=np.int32), np.diff(topo.far_off)),
                                                   topo.far_src, p, dtype, dev)
-        xd = torch.as_tensor(x_host, device=dev)
-        md = torch.as_tensor(m_host, device=dev)
         if self.adaptive_order:
             self._prepare_gears(state, xd, md)
         self.refresh_prepared_state(state, xd, md)
