@@ -138,9 +138,10 @@ int32_t jfm_tree_leaf_geometry(void* stream, int32_t L, int32_t I, const double*
 int32_t jfm_tree_level_geometry(void* stream, int32_t count, const int32_t* level_nodes, const double* pos,
                                 const double* mass, const int32_t* perm, const int32_t* left, const int32_t* right,
                                 const int32_t* node_range, double* node_mass, double* center);
-/* radius [nodes] = max distance from the centre to the node's particles. */
-int32_t jfm_tree_radius(void* stream, int32_t nodes, const double* pos, const int32_t* perm,
-                        const int32_t* node_range, const double* center, double* radius);
+/* radius [nodes] = max distance from the centre to the node's particles;
+ * scratch holds n * 3 doubles (the positions in tree order). */
+int32_t jfm_tree_radius(void* stream, int32_t nodes, int64_t n, const double* pos, const int32_t* perm,
+                        const int32_t* node_range, const double* center, double* scratch, double* radius);
 /* One breadth-first step of the dual-tree walk: expands items_in [n_in][2]
  * into items_out, appending far pairs (target << 32 | source node) and near
  * pairs (target << 32 | source leaf) after counts[1] / counts[2] entries;

@@ -53,7 +53,7 @@ _SIGNATURES = {
                                   c_void_p]),
     "jfm_tree_leaf_geometry": (c_int32, [c_void_p, c_int32, c_int32] + [c_void_p] * 6),
     "jfm_tree_level_geometry": (c_int32, [c_void_p, c_int32] + [c_void_p] * 9),
-    "jfm_tree_radius": (c_int32, [c_void_p, c_int32] + [c_void_p] * 5),
+    "jfm_tree_radius": (c_int32, [c_void_p, c_int32, c_int64] + [c_void_p] * 6),
     "jfm_tree_walk_step": (c_int32, [c_void_p, c_int32, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
                                      c_double, c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_int64, c_void_p]),
     "jfm_keys_to_csr": (c_int32, [c_void_p, c_void_p, c_int64, c_int32, c_void_p, c_void_p]),

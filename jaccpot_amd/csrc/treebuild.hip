@@ -285,21 +285,62 @@ __global__ void __launch_bounds__(256) internal_geometry_kernel(Particles P, int
   }
 }
 
-// one wave per node: max squared distance over the node's particles
-__global__ void __launch_bounds__(256) radius_kernel(Particles P, int32_t nodes, const int32_t* __restrict__ range,
-                                                     const double* __restrict__ center, double* __restrict__ radius) {
+// tree-order copy of the positions, so the radius sweeps read contiguously
+__global__ void __launch_bounds__(256) sorted_positions_kernel(Particles P, int64_t n, double* __restrict__ xs) {
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  for (int a = 0; a < 3; ++a) xs[3 * i + a] = P.x(i, a);
+}
+
+constexpr int kSmallRadius = 2048;  // nodes up to this many particles: one wave each
+
+__device__ inline double dist2(const double* __restrict__ xs, int64_t i, double cx, double cy, double cz) {
+  const double dx = xs[3 * i] - cx, dy = xs[3 * i + 1] - cy, dz = xs[3 * i + 2] - cz;
+  return dx * dx + dy * dy + dz * dz;
+}
+
+// one wave per node of at most kSmallRadius particles: max squared distance
+__global__ void __launch_bounds__(256) radius_small_kernel(const double* __restrict__ xs, int32_t nodes,
+                                                           const int32_t* __restrict__ range,
+                                                           const double* __restrict__ center,
+                                                           double* __restrict__ radius) {
   const int32_t node = blockIdx.x * 4 + threadIdx.x / 64;
   const int lane = threadIdx.x % 64;
   if (node >= nodes) return;  // whole waves leave together
   const int64_t a = range[2 * node], b = range[2 * node + 1];
+  if (b - a + 1 > kSmallRadius) return;
   const double cx = center[3 * node], cy = center[3 * node + 1], cz = center[3 * node + 2];
   double best = 0.0;
-  for (int64_t i = a + lane; i <= b; i += 64) {
-    const double dx = P.x(i, 0) - cx, dy = P.x(i, 1) - cy, dz = P.x(i, 2) - cz;
-    best = fmax(best, dx * dx + dy * dy + dz * dz);
-  }
+  for (int64_t i = a + lane; i <= b; i += 64) best = fmax(best, dist2(xs, i, cx, cy, cz));
   for (int off = 32; off >= 1; off >>= 1) best = fmax(best, __shfl_xor(best, off));
   if (lane == 0) radius[node] = sqrt(best);
+}
+
+// one 1024-thread block per larger node, four independent loads in flight per thread
+__global__ void __launch_bounds__(1024) radius_big_kernel(const double* __restrict__ xs, int32_t nodes,
+                                                          const int32_t* __restrict__ range,
+                                                          const double* __restrict__ center,
+                                                          double* __restrict__ radius) {
+  __shared__ double red[16];
+  const int32_t node = blockIdx.x;
+  const int64_t a = range[2 * node], b = range[2 * node + 1];
+  if (b - a + 1 <= kSmallRadius) return;  // uniform per block
+  const double cx = center[3 * node], cy = center[3 * node + 1], cz = center[3 * node + 2];
+  double best = 0.0;
+  int64_t i = a + threadIdx.x;
+  for (; i + 3 * 1024 <= b; i += 4 * 1024) {
+    const double d0 = dist2(xs, i, cx, cy, cz), d1 = dist2(xs, i + 1024, cx, cy, cz);
+    const double d2 = dist2(xs, i + 2048, cx, cy, cz), d3 = dist2(xs, i + 3072, cx, cy, cz);
+    best = fmax(best, fmax(fmax(d0, d1), fmax(d2, d3)));
+  }
+  for (; i <= b; i += 1024) best = fmax(best, dist2(xs, i, cx, cy, cz));
+  for (int off = 32; off >= 1; off >>= 1) best = fmax(best, __shfl_xor(best, off));
+  if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 16; ++w) best = fmax(best, red[w]);
+    radius[node] = sqrt(best);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -316,19 +357,21 @@ struct WalkOut {
   unsigned long long* count;  // [3]: items, far, near
 };
 
-__device__ inline void push_items(const WalkOut& o, int2 x, int2 y, int2 z, int k) {
-  const int64_t at = static_cast<int64_t>(atomicAdd(&o.count[0], static_cast<unsigned long long>(k)));
-  if (at + k > o.cap_items) return;
-  o.items[at] = x;
-  if (k > 1) o.items[at + 1] = y;
-  if (k > 2) o.items[at + 2] = z;
-}
-
-__device__ inline void push_pair(uint64_t* buf, int64_t cap, unsigned long long* count, uint32_t a, uint32_t b) {
-  const int64_t at = static_cast<int64_t>(atomicAdd(count, 2ull));
-  if (at + 2 > cap) return;
-  buf[at] = (uint64_t(a) << 32) | b;
-  buf[at + 1] = (uint64_t(b) << 32) | a;
+// Reserve `mine` slots of a shared counter for every active lane with one
+// atomic per wave; returns this lane's first slot.
+__device__ inline int64_t wave_reserve(unsigned long long* counter, uint32_t mine) {
+  const int lane = threadIdx.x % 64;
+  const uint64_t active = __ballot(1);
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  // mine is one of 0, 2, 3: two ballots give every lane's exclusive prefix
+  const uint64_t b2 = __ballot(mine == 2), b3 = __ballot(mine == 3);
+  const uint32_t before = 2u * __popcll(b2 & lt) + 3u * __popcll(b3 & lt);
+  const uint32_t total = 2u * __popcll(b2) + 3u * __popcll(b3);
+  const int leader = __ffsll(static_cast<long long>(active)) - 1;
+  unsigned long long base = 0;
+  if (lane == leader && total) base = atomicAdd(counter, static_cast<unsigned long long>(total));
+  base = __shfl(base, leader);
+  return static_cast<int64_t>(base) + before;
 }
 
 __global__ void __launch_bounds__(256) walk_step_kernel(const int2* __restrict__ in, int64_t n_in, int32_t I,
@@ -337,31 +380,54 @@ __global__ void __launch_bounds__(256) walk_step_kernel(const int2* __restrict__
                                                         const double* __restrict__ C, const double* __restrict__ R,
                                                         double th2, WalkOut o) {
   const int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (t >= n_in) return;
+  if (t >= n_in) return;  // the lanes that stay all take part in the reservations below
   const int32_t a = in[t].x, b = in[t].y;
+  uint32_t n_items = 0, n_far = 0, n_near = 0;
+  int2 it0 = make_int2(0, 0), it1 = it0, it2 = it0;
   if (a == b) {
-    if (a >= I) return;
-    const int32_t la = left[a], ra = right[a];
-    push_items(o, make_int2(la, ra), make_int2(ra, ra), make_int2(la, la), 3);
-    return;
+    if (a < I) {
+      const int32_t la = left[a], ra = right[a];
+      it0 = make_int2(la, ra);
+      it1 = make_int2(ra, ra);
+      it2 = make_int2(la, la);
+      n_items = 3;
+    }
+  } else {
+    const double dx = C[3 * a] - C[3 * b], dy = C[3 * a + 1] - C[3 * b + 1], dz = C[3 * a + 2] - C[3 * b + 2];
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    const double s = R[a] + R[b];
+    const bool al = a >= I, bl = b >= I;
+    if (d2 > 0.0 && s * s <= th2 * d2) {
+      n_far = 2;
+    } else if (al && bl) {
+      n_near = 2;
+    } else if (bl || (!al && R[a] >= R[b])) {
+      it0 = make_int2(right[a], b);
+      it1 = make_int2(left[a], b);
+      n_items = 2;
+    } else {
+      it0 = make_int2(a, right[b]);
+      it1 = make_int2(a, left[b]);
+      n_items = 2;
+    }
   }
-  const double dx = C[3 * a] - C[3 * b], dy = C[3 * a + 1] - C[3 * b + 1], dz = C[3 * a + 2] - C[3 * b + 2];
-  const double d2 = dx * dx + dy * dy + dz * dz;
-  const double s = R[a] + R[b];
-  if (d2 > 0.0 && s * s <= th2 * d2) {
-    push_pair(o.far, o.cap_far, &o.count[1], uint32_t(a), uint32_t(b));
-    return;
+  const int64_t at_i = wave_reserve(&o.count[0], n_items);
+  const int64_t at_f = wave_reserve(&o.count[1], n_far);
+  const int64_t at_n = wave_reserve(&o.count[2], n_near);
+  if (n_items && at_i + n_items <= o.cap_items) {
+    o.items[at_i] = it0;
+    o.items[at_i + 1] = it1;
+    if (n_items == 3) o.items[at_i + 2] = it2;
   }
-  const bool al = a >= I, bl = b >= I;
-  if (al && bl) {
-    push_pair(o.near, o.cap_near, &o.count[2], uint32_t(a - I), uint32_t(b - I));
-    return;
+  if (n_far && at_f + 2 <= o.cap_far) {
+    o.far[at_f] = (uint64_t(uint32_t(a)) << 32) | uint32_t(b);
+    o.far[at_f + 1] = (uint64_t(uint32_t(b)) << 32) | uint32_t(a);
   }
-  const bool split_a = bl || (!al && R[a] >= R[b]);
-  if (split_a)
-    push_items(o, make_int2(right[a], b), make_int2(left[a], b), make_int2(0, 0), 2);
-  else
-    push_items(o, make_int2(a, right[b]), make_int2(a, left[b]), make_int2(0, 0), 2);
+  if (n_near && at_n + 2 <= o.cap_near) {
+    const uint32_t ta = uint32_t(a - I), tb = uint32_t(b - I);
+    o.near[at_n] = (uint64_t(ta) << 32) | tb;
+    o.near[at_n + 1] = (uint64_t(tb) << 32) | ta;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -513,13 +579,17 @@ int32_t jfm_tree_level_geometry(void* stream, int32_t count, const int32_t* leve
   return jfm::check_launch("jfm_tree_level_geometry");
 }
 
-int32_t jfm_tree_radius(void* stream, int32_t nodes, const double* pos, const int32_t* perm,
-                        const int32_t* node_range, const double* center, double* radius) {
-  if (nodes < 1 || !pos || !perm || !node_range || !center || !radius)
+int32_t jfm_tree_radius(void* stream, int32_t nodes, int64_t n, const double* pos, const int32_t* perm,
+                        const int32_t* node_range, const double* center, double* scratch, double* radius) {
+  if (nodes < 1 || n < 1 || !pos || !perm || !node_range || !center || !scratch || !radius)
     return fail(JFM_ERR_ARG, "jfm_tree_radius: bad argument");
+  hipStream_t s = jfm::as_stream(stream);
   jfm::Particles P{pos, nullptr, perm};
-  hipLaunchKernelGGL(jfm::radius_kernel, dim3(jfm::grid_for(nodes, 4)), dim3(256), 0, jfm::as_stream(stream), P,
-                     nodes, node_range, center, radius);
+  hipLaunchKernelGGL(jfm::sorted_positions_kernel, dim3(jfm::grid_for(n)), dim3(256), 0, s, P, n, scratch);
+  hipLaunchKernelGGL(jfm::radius_small_kernel, dim3(jfm::grid_for(nodes, 4)), dim3(256), 0, s, scratch, nodes,
+                     node_range, center, radius);
+  hipLaunchKernelGGL(jfm::radius_big_kernel, dim3(nodes), dim3(1024), 0, s, scratch, nodes, node_range, center,
+                     radius);
   return jfm::check_launch("jfm_tree_radius");
 }
 

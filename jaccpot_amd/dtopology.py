@@ -152,14 +152,16 @@ def build_topology_device(x: torch.Tensor, m: torch.Tensor, *, leaf_size: int, t
             nat.call("jfm_tree_level_geometry", s, b - a, m2m_nodes[a:].data_ptr(), x.data_ptr(), m.data_ptr(),
                      perm.data_ptr(), left.data_ptr(), right.data_ptr(), node_range.data_ptr(), mass.data_ptr(),
                      center.data_ptr())
-    nat.call("jfm_tree_radius", s, nodes, x.data_ptr(), perm.data_ptr(), node_range.data_ptr(), center.data_ptr(),
-             radius.data_ptr())
+    xs = torch.empty((n, 3), dtype=torch.float64, device=dev)
+    nat.call("jfm_tree_radius", s, nodes, n, x.data_ptr(), perm.data_ptr(), node_range.data_ptr(), center.data_ptr(),
+             xs.data_ptr(), radius.data_ptr())
+    del xs
 
     # ---- dual-tree walk, breadth first -----------------------------------------------------
-    items = [_Grow(max(1024, 2 * L), torch.int32, dev, 2), _Grow(max(1024, 2 * L), torch.int32, dev, 2)]
+    items = [_Grow(max(1024, 8 * L), torch.int32, dev, 2), _Grow(max(1024, 8 * L), torch.int32, dev, 2)]
     items[0].buf[0].zero_()  # the root self item (0, 0)
-    far = _Grow(64 * L, torch.int64, dev)
-    near = _Grow(128 * L, torch.int64, dev)
+    far = _Grow(max(1024, 128 * L), torch.int64, dev)
+    near = _Grow(max(1024, 320 * L), torch.int64, dev)
     counts = torch.zeros(3, dtype=torch.int64, device=dev)
     n_in, n_far, n_near = 1, 0, 0
     while n_in > 0:
