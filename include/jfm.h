@@ -158,6 +158,32 @@ int32_t jfm_keys_to_csr(void* stream, const uint64_t* sorted_keys, int64_t K, in
 /* Number of significant bits of max_key (sort widths). */
 int32_t jfm_key_bits(uint64_t max_key);
 
+/* ------------------------------------------------------------------------
+ * Prepare-time list surgery (the reference's per-order pair bucketing before
+ * its M2L launches, runtime/_interaction_cache.py p_gears groups, and the
+ * flat pair layout behind jfm_m2l_flat).
+ * ---------------------------------------------------------------------- */
+/* row_of [K]: the row of every entry of a CSR with off [rows+1]. */
+int32_t jfm_pair_rows(void* stream, int32_t rows, const int32_t* off, int64_t K, int32_t* row_of);
+/* keys [K] = (slot << 32 | k), slot = index of gear_of_pair[k] in gears3[0..2];
+ * flags |= 1 if a gear is not one of them. */
+int32_t jfm_gear_keys(void* stream, int64_t K, const int8_t* gear_of_pair, const int32_t* gears3, uint64_t* keys,
+                      int32_t* flags);
+/* out_tgt[i] = row_of[index[i]], out_src[i] = src[index[i]] for i < n. */
+int32_t jfm_gather_pairs(void* stream, int64_t n, const int32_t* index, const int32_t* row_of, const int32_t* src,
+                         int32_t* out_tgt, int32_t* out_src);
+/* keys[i] = tgt[i] << 32 | src[i]. */
+int32_t jfm_pair_keys(void* stream, int64_t n, const int32_t* tgt, const int32_t* src, uint64_t* keys);
+/* Appends every row r with off[r+1] > off[r] to out (rows entries of room),
+ * *count (zeroed by the caller) += appended; order unspecified. */
+int32_t jfm_nonempty_rows(void* stream, int32_t rows, const int32_t* off, uint64_t* out, uint64_t* count);
+/* Targets whose pairs (pair_tgt [K], grouped by target) cross a 64-pair block
+ * boundary: span_tgt, first block span_w0, last block span_w1 (room for
+ * (K-1)/64 entries), *count (zeroed by the caller) = number found; order
+ * unspecified (jfm_m2l_flat's fix-up treats every span on its own). */
+int32_t jfm_flat_spans(void* stream, int64_t K, const int32_t* pair_tgt, int32_t* span_tgt, int32_t* span_w0,
+                       int32_t* span_w1, uint64_t* count);
+
 /* Dehnen real multipole swap matrices B_U for degrees 0..p, float64, packed
  * row-major block after block (sum (2l+1)^2 entries); the constants every
  * rotation kernel is compiled with (jaccpot/operators/real_dehnen_q.py:343-470). */

@@ -58,6 +58,12 @@ _SIGNATURES = {
                                      c_double, c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_int64, c_void_p]),
     "jfm_keys_to_csr": (c_int32, [c_void_p, c_void_p, c_int64, c_int32, c_void_p, c_void_p]),
     "jfm_key_bits": (c_int32, [ctypes.c_uint64]),
+    "jfm_pair_rows": (c_int32, [c_void_p, c_int32, c_void_p, c_int64, c_void_p]),
+    "jfm_gear_keys": (c_int32, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "jfm_gather_pairs": (c_int32, [c_void_p, c_int64] + [c_void_p] * 5),
+    "jfm_pair_keys": (c_int32, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p]),
+    "jfm_nonempty_rows": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_void_p]),
+    "jfm_flat_spans": (c_int32, [c_void_p, c_int64] + [c_void_p] * 5),
     "jfm_tables_bu": (c_int32, [c_int32, c_void_p, c_int64]),
     "jfm_gather_sorted": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_int32, c_void_p]),
     "jfm_p2m": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_int32, c_int32, c_int32, c_int32, c_void_p,

@@ -3,7 +3,9 @@
 A CSR far list (pairs grouped by target) is kept as two device arrays, the
 target and the source of every pair, plus the list of targets whose pairs
 cross a 64-pair block boundary (their per-block partial sums are added in
-block order by the fix-up pass) and the per-block scratch rows.
+block order by the fix-up pass; each such target is handled on its own, so
+the order of that list does not change any result) and the per-block
+scratch rows.
 """
 
 from __future__ import annotations
@@ -44,31 +46,49 @@ class FlatList:
     scratch: torch.Tensor
 
 
-def build_flat_list(tgt: np.ndarray, src: np.ndarray, gear: int, dtype: torch.dtype, device) -> FlatList:
-    """``tgt``/``src`` [K]: target and source node of every pair, grouped by target."""
-    tgt = np.ascontiguousarray(tgt, np.int32)
-    src = np.ascontiguousarray(src, np.int32)
-    K = int(tgt.shape[0])
-    if K and np.any(np.diff(tgt) < 0):
+def _as_device_i32(a, device) -> torch.Tensor:
+    if isinstance(a, torch.Tensor):
+        return a.to(device=device, dtype=torch.int32).contiguous()
+    return torch.as_tensor(np.ascontiguousarray(a, np.int32)).to(device)
+
+
+def build_flat_list(tgt, src, gear: int, dtype: torch.dtype, device) -> FlatList:
+    """``tgt``/``src`` [K] (numpy arrays or device tensors): target and source node
+    of every pair, grouped by target in ascending order.  The spanning targets
+    are found on the device (``jfm_flat_spans``)."""
+    pair_tgt = _as_device_i32(tgt, device)
+    pair_src = _as_device_i32(src, device)
+    K = int(pair_tgt.numel())
+    if K != int(pair_src.numel()):
+        raise ValueError("tgt and src must have the same length")
+    if K > 1 and bool((pair_tgt[1:] < pair_tgt[:-1]).any()):
         raise ValueError("pairs must be grouped by target in ascending order")
-    span_t = span_w0 = span_w1 = np.zeros(0, np.int32)
-    if K:
-        starts = np.concatenate([[0], np.nonzero(np.diff(tgt))[0] + 1])
-        ends = np.concatenate([starts[1:], [K]]) - 1
-        w0 = starts // BLOCK
-        w1 = ends // BLOCK
-        sp = w1 > w0
-        span_t, span_w0, span_w1 = tgt[starts[sp]], w0[sp].astype(np.int32), w1[sp].astype(np.int32)
     n_scr = int(nat.load().jfm_m2l_flat_scratch_size(int(gear), K))
     if n_scr < 0:
         raise ValueError(f"unsupported gear {gear}")
-
-    def dev(a):
-        return torch.as_tensor(a if a.size else np.zeros(1, np.int32)).to(device)
-
-    return FlatList(gear=int(gear), num_pairs=K, pair_tgt=dev(tgt), pair_src=dev(src), num_span=int(span_t.size),
-                    span_tgt=dev(span_t), span_w0=dev(span_w0), span_w1=dev(span_w1),
+    cap = max((K - 1) // BLOCK, 0) + 1
+    span_tgt = torch.empty(cap, dtype=torch.int32, device=device)
+    span_w0 = torch.empty(cap, dtype=torch.int32, device=device)
+    span_w1 = torch.empty(cap, dtype=torch.int32, device=device)
+    count = torch.zeros(1, dtype=torch.int64, device=device)
+    if K:
+        nat.call("jfm_flat_spans", nat.current_stream(), K, pair_tgt.data_ptr(), span_tgt.data_ptr(),
+                 span_w0.data_ptr(), span_w1.data_ptr(), count.data_ptr())
+    num_span = int(count.item())
+    one = torch.zeros(1, dtype=torch.int32, device=device)
+    return FlatList(gear=int(gear), num_pairs=K, pair_tgt=pair_tgt if K else one, pair_src=pair_src if K else one,
+                    num_span=num_span, span_tgt=span_tgt, span_w0=span_w0, span_w1=span_w1,
                     scratch=torch.empty(max(n_scr, 1), dtype=dtype, device=device))
+
+
+def pair_targets(state) -> torch.Tensor:
+    """Target node of every far pair of ``state`` (device int32, CSR order)."""
+    K = state.topology.num_far_pairs
+    row_of = torch.empty(max(K, 1), dtype=torch.int32, device=state.device)
+    if K:
+        nat.call("jfm_pair_rows", nat.current_stream(), state.topology.num_nodes, state.far_off.data_ptr(), K,
+                 row_of.data_ptr())
+    return row_of[:K]
 
 
 def launch_flat(fl: FlatList, stream, code: int, p: int, multipoles: torch.Tensor, centers4: torch.Tensor,

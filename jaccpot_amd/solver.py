@@ -25,7 +25,7 @@ import numpy as np
 import torch
 
 from . import _native as nat
-from .flatlist import build_flat_list, launch_flat, prefers_flat
+from .flatlist import build_flat_list, launch_flat, pair_targets, prefers_flat
 from .dtopology import build_topology_device
 from .topology import Topology, build_topology
 
@@ -258,8 +258,8 @@ class FastMultipoleMethod:
         if self.mixed_precision and dtype == torch.float32:
             state.extra["mixed"] = self._mixed_schedule(topo, p, dev)
         if prefers_flat(topo.num_far_pairs, int(topo.m2l_targets.size)):
-            state.extra["flat"] = build_flat_list(np.repeat(np.arange(nodes, dtype=np.int32), np.diff(topo.far_off)),
-                                                  topo.far_src, p, dtype, dev)
+            state.extra["flat"] = build_flat_list(pair_targets(state), state.far_src[: topo.num_far_pairs], p, dtype,
+                                                  dev)
         if self.adaptive_order:
             self._prepare_gears(state, xd, md)
         self.refresh_prepared_state(state, xd, md)
@@ -284,9 +284,9 @@ class FastMultipoleMethod:
         if K == 0:
             return
         X = pair_features(state)
-        g = self.order_model.select_gears(X, self.gear_tolerance, state).cpu().numpy()
-        state.extra["gear_of_pair"] = g
+        g = self.order_model.select_gears(X, self.gear_tolerance, state)
         state.extra["gears"] = build_gear_schedule(state, g, gears)
+        state.extra["gear_of_pair"] = g.cpu().numpy()
 
     def refresh_prepared_state(self, state: FMMPreparedState, positions: torch.Tensor, masses: torch.Tensor) -> None:
         """Re-run gather + upward + downward at the state's frozen topology.

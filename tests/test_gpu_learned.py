@@ -252,3 +252,48 @@ def test_shipped_gear368_model_cuts_order8_work(require_gpu):
     e6 = err(plain.evaluate_prepared_state(st))
     print(f"[gpu]   all pairs at order 6: err {e6:.3e}", flush=True)
     assert el < e6
+
+
+def test_gear_schedule_matches_host_split(require_gpu):
+    """The device gear split (radix sort by gear, CSR per gear, targets, flat
+    lists and their spanning targets) equals a NumPy split of the far CSR."""
+    from jaccpot_amd import FastMultipoleMethod
+    from jaccpot_amd.flatlist import BLOCK, build_flat_list
+    from jaccpot_amd.learned import build_gear_schedule
+
+    print("[gpu] gear schedule split", flush=True)
+    x, m = ics.plummer(6000, 9)
+    fmm = FastMultipoleMethod(precision="fp32", theta=0.5, softening=1e-3)
+    state = fmm.prepare_state(x.astype(np.float32), m.astype(np.float32), leaf_size=4, max_order=6)
+    topo = state.topology
+    K, nodes = topo.num_far_pairs, topo.num_nodes
+    g = np.random.default_rng(5).choice([2, 4, 6], size=K, p=[0.6, 0.3, 0.1]).astype(np.int8)
+    sched = build_gear_schedule(state, g, (2, 4, 6))
+    tgt = np.repeat(np.arange(nodes), np.diff(topo.far_off))
+
+    def host_spans(t):
+        if t.size == 0:
+            return set()
+        starts = np.concatenate([[0], np.nonzero(np.diff(t))[0] + 1])
+        ends = np.concatenate([starts[1:], [t.size]]) - 1
+        w0, w1 = starts // BLOCK, ends // BLOCK
+        sp = w1 > w0
+        return set(zip(t[starts[sp]].tolist(), w0[sp].tolist(), w1[sp].tolist()))
+
+    for gi, gear in enumerate((2, 4, 6)):
+        sel = g == gear
+        t, s = tgt[sel], topo.far_src[sel]
+        cnt = np.bincount(t, minlength=nodes)
+        off = np.concatenate([[0], np.cumsum(cnt)])
+        tl = np.nonzero(cnt)[0]
+        assert sched.counts[gear] == int(sel.sum())
+        assert sched.num_targets[gi] == tl.size
+        assert np.array_equal(sched.offsets[gi].cpu().numpy(), off)
+        assert np.array_equal(sched.sources[gi].cpu().numpy()[: t.size], s)
+        assert np.array_equal(sched.targets[gi].cpu().numpy()[: tl.size], tl)
+        fl = build_flat_list(t, s, gear, torch.float32, "cuda")
+        assert np.array_equal(fl.pair_tgt.cpu().numpy()[: t.size], t)
+        assert np.array_equal(fl.pair_src.cpu().numpy()[: t.size], s)
+        got = set(zip(*(a.cpu().numpy()[: fl.num_span].tolist() for a in (fl.span_tgt, fl.span_w0, fl.span_w1))))
+        assert got == host_spans(t) and fl.num_span == len(got)
+        print(f"[gpu]   gear {gear}: {t.size} pairs, {tl.size} targets, {fl.num_span} spanning targets", flush=True)
