@@ -5,9 +5,10 @@ builders for the reference, jaccpot/runtime/fmm_state.py:630-800,
 runtime/_interaction_cache.py:523,564), run as libjfm device kernels
 (``csrc/treebuild.hip``): bounds, Morton codes, stable radix sort, radix tree,
 levels, centres of mass and radii, breadth-first dual-tree walk, CSR lists.
-Returns the same ``Topology`` (host arrays, so every host consumer works
-unchanged) with the device copies in ``extra["device"]`` so that
-``prepare_state`` does not upload them again.
+Returns a ``Topology`` (host arrays, so every host consumer works unchanged;
+the three largest are copied from the device on first read) with the device
+arrays in ``extra["device"]`` so that ``prepare_state`` does not upload them
+again.
 """
 
 from __future__ import annotations
@@ -20,6 +21,35 @@ import torch
 
 from . import _native as nat
 from .topology import Topology
+
+
+class _Lazy:
+    """A device array whose host copy is made on first access."""
+
+    __slots__ = ("t",)
+
+    def __init__(self, t: torch.Tensor):
+        self.t = t
+
+
+class DeviceTopology(Topology):
+    """``Topology`` whose largest host arrays (perm, far_src, near_src) are copied
+    from the device only when first read; the device build never needs them."""
+
+    def __getattribute__(self, name):
+        v = object.__getattribute__(self, name)
+        if type(v) is _Lazy:
+            v = v.t.cpu().numpy()
+            object.__setattr__(self, name, v)
+        return v
+
+    @property
+    def num_far_pairs(self) -> int:
+        return int(self.extra["num_far_pairs"])
+
+    @property
+    def num_near_entries(self) -> int:
+        return int(self.extra["num_near_entries"])
 
 
 def _sort_u64(keys: torch.Tensor, end_bit: int, vals: torch.Tensor | None = None):
@@ -205,16 +235,19 @@ def build_topology_device(x: torch.Tensor, m: torch.Tensor, *, leaf_size: int, t
         return t.cpu().numpy()
 
     far_off_h = h(far_off)
-    topo = Topology(
+    topo = DeviceTopology(
         num_particles=n, leaf_size=int(leaf_size), num_leaves=L, num_internal=I, num_nodes=nodes, max_depth=D,
-        theta=float(theta), perm=h(perm), parent=h(parent), left=h(left[:I]), right=h(right[:I]),
+        theta=float(theta), perm=_Lazy(perm), parent=h(parent), left=h(left[:I]), right=h(right[:I]),
         node_range=h(node_range), depth=h(depth), center=h(center), radius=h(radius), mass=h(mass),
-        far_off=far_off_h, far_src=h(far_src), near_off=h(near_off), near_src=h(near_src),
+        far_off=far_off_h, far_src=_Lazy(far_src), near_off=h(near_off), near_src=_Lazy(near_src),
         m2m_nodes=h(m2m_nodes), m2m_off=m2m_off.astype(np.int32), l2l_nodes=h(l2l_nodes),
         l2l_off=l2l_off.astype(np.int32),
         m2l_targets=np.nonzero(np.diff(far_off_h) > 0)[0].astype(np.int32),
-        build_seconds=time.perf_counter() - t0,
+        build_seconds=0.0,
     )
+    topo.extra["num_far_pairs"] = n_far
+    topo.extra["num_near_entries"] = n_near
+    topo.build_seconds = time.perf_counter() - t0
     topo.extra["device"] = device
     topo.extra["device_build_seconds"] = t1 - t0
     return topo

@@ -231,25 +231,22 @@ class FastMultipoleMethod:
         code = nat.dtype_code(dtype)
         cs = nat.coeff_stride(p, dtype)
 
-        def i32(a, name=None):
-            d = on_dev.get(name) if name else None
+        def i32(name):
+            d = on_dev.get(name)
             if d is not None and d.numel() > 0:
                 return d
-            a = np.asarray(a)
+            a = np.asarray(getattr(topo, name))
             return torch.as_tensor(np.ascontiguousarray(a if a.size else np.zeros(1), np.int32)).to(dev)
 
         nodes = topo.num_nodes
         state = FMMPreparedState(
             topology=topo, order=p, dtype=dtype, G=self.G, softening=self.softening, theta=th, device=dev,
             input_kind=kind, input_dtype=in_dtype, input_device=in_dev, cs=cs,
-            perm=i32(topo.perm, "perm"),
+            perm=i32("perm"),
             pos4=torch.empty((topo.num_particles, 4), dtype=dtype, device=dev),
-            node_range=i32(topo.node_range, "node_range"), left=i32(topo.left, "left"),
-            right=i32(topo.right, "right"), parent=i32(topo.parent, "parent"),
-            m2m_nodes=i32(topo.m2m_nodes, "m2m_nodes"), l2l_nodes=i32(topo.l2l_nodes, "l2l_nodes"),
-            m2l_targets=i32(topo.m2l_targets),
-            far_off=i32(topo.far_off, "far_off"), far_src=i32(topo.far_src, "far_src"),
-            near_off=i32(topo.near_off, "near_off"), near_src=i32(topo.near_src, "near_src"),
+            node_range=i32("node_range"), left=i32("left"), right=i32("right"), parent=i32("parent"),
+            m2m_nodes=i32("m2m_nodes"), l2l_nodes=i32("l2l_nodes"), m2l_targets=i32("m2l_targets"),
+            far_off=i32("far_off"), far_src=i32("far_src"), near_off=i32("near_off"), near_src=i32("near_src"),
             centers4=torch.zeros((nodes, 4), dtype=dtype, device=dev),
             multipoles=torch.zeros((nodes, cs), dtype=dtype, device=dev),
             locals=torch.zeros((nodes, cs), dtype=dtype, device=dev),
@@ -311,7 +308,7 @@ class FastMultipoleMethod:
         """Shadow rows and level tables for the float64 top three levels."""
         I = topo.num_internal
         depth = topo.depth
-        top = np.array([k for k in range(I) if depth[k] <= 2], np.int32)
+        top = np.nonzero(depth[:I] <= 2)[0].astype(np.int32)
         shadow = np.full(topo.num_nodes, -1, np.int32)
         shadow[top] = np.arange(top.size, dtype=np.int32)
         m2m_levels = [top[depth[top] == d] for d in (2, 1, 0)]
