@@ -136,7 +136,7 @@ def main() -> int:
         torch.cuda.synchronize()
         prep_s = time.perf_counter() - t0
         topo = state.topology
-        log(f"[{tag}] prepared in {prep_s:.2f}s (host topology {topo.build_seconds:.2f}s): L={topo.num_leaves} "
+        log(f"[{tag}] prepared in {prep_s:.2f}s (topology build {topo.build_seconds:.2f}s): L={topo.num_leaves} "
             f"nodes={topo.num_nodes} depth={topo.max_depth} far pairs={topo.num_far_pairs} "
             f"near={topo.num_near_entries}")
         dev = state.device
