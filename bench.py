@@ -164,9 +164,7 @@ def main() -> int:
                      acc.data_ptr(), None)
             if ev is not None:
                 ev[3].record()
-            nat.call("jfm_p2p", s, code, state.pos4.data_ptr(), N, topo.num_leaves, leaf_size,
-                     state.near_off.data_ptr(), state.near_src.data_ptr(), eps * eps, state.G,
-                     state.perm.data_ptr(), acc.data_ptr(), None, flags, 1)
+            fmm._p2p_device(state, acc, None, flags, accumulate=True)
             if ev is not None:
                 ev[4].record()
 

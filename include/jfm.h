@@ -289,6 +289,27 @@ int32_t jfm_p2p(void* stream, int32_t dtype, const void* pos4, int32_t n, int32_
                 const int32_t* near_off, const int32_t* near_src, double eps2, double G, const int32_t* out_index,
                 void* acc, void* pot, int32_t flags, int32_t accumulate);
 
+/* P2P over work items that split long near lists (the reference's bucketed
+ * near-field launches, runtime/kernels/nearfield/_kernels.py:40-338, cut
+ * into equal slot ranges so no leaf's sources serialise on one workgroup).
+ * Item k covers slots [item_s0[k], item_s1[k]) of leaf item_leaf[k]'s
+ * sequence (self leaf, then its near leaves, leaf_size slots each);
+ * item_part[k] < 0: the item is its leaf's only one and writes (or adds to,
+ * accumulate != 0) the output rows like jfm_p2p; otherwise it writes its
+ * targets' unscaled partial sums to partials[(item_part[k] * leaf_size + i) * 4
+ * + {0..3}] (num_part rows of room).  jfm_p2p_reduce then adds, for every
+ * split leaf, the partials split_p0[k] .. split_p1[k]-1 in that order
+ * (compensated) and writes or adds the output rows. */
+int32_t jfm_p2p_items(void* stream, int32_t dtype, const void* pos4, int32_t n, int32_t num_leaves, int32_t leaf_size,
+                      int32_t num_items, const int32_t* item_leaf, const int32_t* item_s0, const int32_t* item_s1,
+                      const int32_t* item_part, int32_t num_part, void* partials, const int32_t* near_off,
+                      const int32_t* near_src, double eps2, double G, const int32_t* out_index, void* acc, void* pot,
+                      int32_t flags, int32_t accumulate);
+int32_t jfm_p2p_reduce(void* stream, int32_t dtype, const void* pos4, int32_t n, int32_t leaf_size, int32_t num_split,
+                       const int32_t* split_leaf, const int32_t* split_p0, const int32_t* split_p1,
+                       const void* partials, double eps2, double G, const int32_t* out_index, void* acc, void* pot,
+                       int32_t flags, int32_t accumulate);
+
 /* Fused leaf evaluation: P2P near field (self leaf i != j plus near leaves) and
  * L2P far field (a = -G grad_delta phi, delta = centre - x), per flags.  Results
  * for tree slot i go to row out_index[i] of acc [rows][3] (and pot [rows]);

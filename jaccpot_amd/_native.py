@@ -86,6 +86,12 @@ _SIGNATURES = {
                           c_double, c_void_p, c_void_p, c_void_p]),
     "jfm_p2p": (c_int32, [c_void_p, c_int32, c_void_p, c_int32, c_int32, c_int32, c_void_p, c_void_p, c_double,
                           c_double, c_void_p, c_void_p, c_void_p, c_int32, c_int32]),
+    "jfm_p2p_items": (c_int32, [c_void_p, c_int32, c_void_p, c_int32, c_int32, c_int32, c_int32, c_void_p, c_void_p,
+                                c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_double, c_double,
+                                c_void_p, c_void_p, c_void_p, c_int32, c_int32]),
+    "jfm_p2p_reduce": (c_int32, [c_void_p, c_int32, c_void_p, c_int32, c_int32, c_int32, c_void_p, c_void_p,
+                                 c_void_p, c_void_p, c_double, c_double, c_void_p, c_void_p, c_void_p, c_int32,
+                                 c_int32]),
     "jfm_evaluate": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_int32, c_int32, c_int32, c_int32, c_void_p,
                                c_void_p, c_void_p, c_void_p, c_double, c_double, c_void_p, c_void_p, c_void_p,
                                c_int32]),

@@ -101,13 +101,29 @@ __device__ __forceinline__ void fetch_positions(const T* __restrict__ pos4, int 
   }
 }
 
+// Work items split the source slots of leaves with long near lists: item k
+// covers slots [s0, s1) of leaf `leaf`; part < 0 -> the item is the leaf's
+// only one and writes the output rows, else it writes the unscaled partial
+// sums of its targets to partials[(part * leaf_size + i) * 4 + c] and
+// p2p_reduce_kernel adds a leaf's partials in item order.  leaf == NULL: one
+// item per leaf covering all of its slots.
+template <typename T>
+struct P2PItems {
+  const int32_t* leaf;
+  const int32_t* s0;
+  const int32_t* s1;
+  const int32_t* part;
+  T* partials;
+  int32_t num_part;  // rows of partials (items with part >= num_part write nothing)
+};
+
 // SAFE: guard r^2 == 0 (unsoftened self term / coincident points).
 template <typename T, int WT, bool POT, bool COMP, bool SAFE>
 __global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos4, int32_t n, int32_t leaf_size,
                                                        const int32_t* __restrict__ near_off,
                                                        const int32_t* __restrict__ near_src, T eps2, T G,
                                                        const int32_t* __restrict__ out_index, T* __restrict__ acc,
-                                                       T* __restrict__ pot, int32_t accumulate) {
+                                                       T* __restrict__ pot, int32_t accumulate, P2PItems<T> items) {
   constexpr int NPART = kThreads / WT;
   constexpr int TILE = sizeof(T) == 4 ? 1024 : 512;
   constexpr int LOADS = TILE / kThreads;
@@ -118,7 +134,8 @@ __global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos
   // each XCD a contiguous run of leaves (neighbouring leaves share most of
   // their source leaves, which then stay in that XCD's L2).  Bijective:
   // XCD x < r owns q+1 leaves, the others q, where nblk = 8q + r.
-  const int leaf = xcd_contiguous(blockIdx.x, gridDim.x);
+  const int item = xcd_contiguous(blockIdx.x, gridDim.x);
+  const int leaf = items.leaf ? items.leaf[item] : item;
   const int i = threadIdx.x % WT;
   const int part = threadIdx.x / WT;
   const int first = leaf * leaf_size;
@@ -126,7 +143,10 @@ __global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos
   const V4<T> xi = load4(pos4, first + (i < cnt ? i : 0));
   const int nb = near_off[leaf];
   const int nsl = 1 + (near_off[leaf + 1] - nb);
-  const int total = nsl * leaf_size;
+  // slots [start, total) of the leaf's sequence (self leaf, then near leaves)
+  const int start = items.leaf ? items.s0[item] : 0;
+  const int total = items.leaf ? items.s1[item] : nsl * leaf_size;
+  const int part_out = items.leaf ? items.part[item] : -1;
 
   // The next tile's sources are fetched into registers while the current
   // tile is being consumed, hiding the global-load latency behind the
@@ -135,13 +155,13 @@ __global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos
   bool ok[LOADS];
   int lst[LOADS];  // near-list entries of the tile after `next`
   const int tid = threadIdx.x;
-  fetch_list<WT, LOADS, kThreads>(leaf_size, near_src + nb, nsl - 1, near_off, tid, lst);
-  fetch_positions<T, WT, LOADS, kThreads>(pos4, n, leaf_size, leaf, total, tid, lst, next, ok);
-  fetch_list<WT, LOADS, kThreads>(leaf_size, near_src + nb, nsl - 1, near_off, TILE + tid, lst);
+  fetch_list<WT, LOADS, kThreads>(leaf_size, near_src + nb, nsl - 1, near_off, start + tid, lst);
+  fetch_positions<T, WT, LOADS, kThreads>(pos4, n, leaf_size, leaf, total, start + tid, lst, next, ok);
+  fetch_list<WT, LOADS, kThreads>(leaf_size, near_src + nb, nsl - 1, near_off, start + TILE + tid, lst);
 
   T sx = 0, sy = 0, sz = 0, sp = 0;
   T kx = 0, ky = 0, kz = 0, kp = 0;
-  for (int base = 0; base < total; base += TILE) {
+  for (int base = start; base < total; base += TILE) {
     __syncthreads();
 #pragma unroll
     for (int l = 0; l < LOADS; ++l) {
@@ -211,6 +231,15 @@ __global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos
     ay -= cy;
     az -= cz;
     ap -= cp;
+    if (part_out >= 0) {
+      if (part_out >= items.num_part) return;
+      T* dst = items.partials + (static_cast<int64_t>(part_out) * leaf_size + i) * 4;
+      dst[0] = ax;
+      dst[1] = ay;
+      dst[2] = az;
+      dst[3] = ap;
+      return;
+    }
     if (POT && eps2 > T(0)) ap -= xi.w * fast_rsqrt(eps2);  // remove the softened self term
     const int64_t row = out_index ? out_index[first + i] : first + i;
     if (accumulate) {
@@ -227,16 +256,54 @@ __global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos
   }
 }
 
+// Leaves split over several items: target i of split leaf k adds the partials
+// of items p0[k] .. p1[k]-1 in order (Kahan), then applies what the single-item
+// path applies (self potential, G, output row).
+template <typename T, bool POT>
+__global__ __launch_bounds__(kThreads) void p2p_reduce_kernel(const T* __restrict__ pos4, int32_t n, int32_t leaf_size,
+                                                              int32_t num_split, const int32_t* __restrict__ split_leaf,
+                                                              const int32_t* __restrict__ p0,
+                                                              const int32_t* __restrict__ p1,
+                                                              const T* __restrict__ partials, T eps2, T G,
+                                                              const int32_t* __restrict__ out_index,
+                                                              T* __restrict__ acc, T* __restrict__ pot,
+                                                              int32_t accumulate) {
+  const int64_t t = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (t >= int64_t(num_split) * leaf_size) return;
+  const int k = static_cast<int>(t / leaf_size), i = static_cast<int>(t % leaf_size);
+  const int first = split_leaf[k] * leaf_size;
+  if (first + i >= n) return;
+  T s[4] = {0, 0, 0, 0}, c[4] = {0, 0, 0, 0};
+  for (int p = p0[k]; p < p1[k]; ++p) {
+    const T* src = partials + (static_cast<int64_t>(p) * leaf_size + i) * 4;
+    for (int q = 0; q < 4; ++q) kahan_add(s[q], c[q], src[q]);
+  }
+  T ap = s[3];
+  if (POT && eps2 > T(0)) ap -= pos4[4 * static_cast<int64_t>(first + i) + 3] * fast_rsqrt(eps2);
+  const int64_t row = out_index ? out_index[first + i] : first + i;
+  if (accumulate) {
+    acc[3 * row + 0] += G * s[0];
+    acc[3 * row + 1] += G * s[1];
+    acc[3 * row + 2] += G * s[2];
+    if (POT) pot[row] += -G * ap;
+  } else {
+    acc[3 * row + 0] = G * s[0];
+    acc[3 * row + 1] = G * s[1];
+    acc[3 * row + 2] = G * s[2];
+    if (POT) pot[row] = -G * ap;
+  }
+}
+
 template <typename T, int WT>
-int32_t launch_p2p_wt(hipStream_t s, const T* pos4, int32_t n, int32_t leaf_size, int32_t num_leaves,
+int32_t launch_p2p_wt(hipStream_t s, const T* pos4, int32_t n, int32_t leaf_size, int32_t num_blocks,
                       const int32_t* near_off, const int32_t* near_src, T eps2, T G, const int32_t* out_index,
-                      T* acc, T* pot, bool comp, bool accumulate) {
+                      T* acc, T* pot, bool comp, bool accumulate, P2PItems<T> items) {
   const bool want_pot = pot != nullptr;
   const bool safe = !(eps2 > T(0));
 #define JFM_P2P_LAUNCH(POT, COMP, SAFE)                                                                  \
-  p2p_kernel<T, WT, POT, COMP, SAFE><<<num_leaves, kThreads, 0, s>>>(pos4, n, leaf_size, near_off, near_src, \
+  p2p_kernel<T, WT, POT, COMP, SAFE><<<num_blocks, kThreads, 0, s>>>(pos4, n, leaf_size, near_off, near_src, \
                                                                      eps2, G, out_index, acc, pot,           \
-                                                                     accumulate ? 1 : 0)
+                                                                     accumulate ? 1 : 0, items)
   if (want_pot) {
     if (comp) {
       if (safe) JFM_P2P_LAUNCH(true, true, true); else JFM_P2P_LAUNCH(true, true, false);
@@ -257,24 +324,24 @@ int32_t launch_p2p_wt(hipStream_t s, const T* pos4, int32_t n, int32_t leaf_size
 template <typename T>
 int32_t launch_p2p(hipStream_t s, const T* pos4, int32_t n, int32_t leaf_size, int32_t num_leaves,
                    const int32_t* near_off, const int32_t* near_src, T eps2, T G, const int32_t* out_index, T* acc,
-                   T* pot, bool comp, bool accumulate) {
+                   T* pot, bool comp, bool accumulate, P2PItems<T> items = P2PItems<T>{}) {
   if (leaf_size <= 8)
     return launch_p2p_wt<T, 8>(s, pos4, n, leaf_size, num_leaves, near_off, near_src, eps2, G, out_index, acc,
-                               pot, comp, accumulate);
+                               pot, comp, accumulate, items);
   if (leaf_size <= 16)
     return launch_p2p_wt<T, 16>(s, pos4, n, leaf_size, num_leaves, near_off, near_src, eps2, G, out_index, acc,
-                                pot, comp, accumulate);
+                                pot, comp, accumulate, items);
   if (leaf_size <= 32)
     return launch_p2p_wt<T, 32>(s, pos4, n, leaf_size, num_leaves, near_off, near_src, eps2, G, out_index, acc,
-                                pot, comp, accumulate);
+                                pot, comp, accumulate, items);
   if (leaf_size <= 64)
     return launch_p2p_wt<T, 64>(s, pos4, n, leaf_size, num_leaves, near_off, near_src, eps2, G, out_index, acc,
-                                pot, comp, accumulate);
+                                pot, comp, accumulate, items);
   if (leaf_size <= 128)
     return launch_p2p_wt<T, 128>(s, pos4, n, leaf_size, num_leaves, near_off, near_src, eps2, G, out_index, acc,
-                                 pot, comp, accumulate);
+                                 pot, comp, accumulate, items);
   return launch_p2p_wt<T, 256>(s, pos4, n, leaf_size, num_leaves, near_off, near_src, eps2, G, out_index, acc,
-                               pot, comp, accumulate);
+                               pot, comp, accumulate, items);
 }
 
 }  // namespace
@@ -307,6 +374,62 @@ int32_t jfm_p2p(void* stream, int32_t dtype, const void* pos4, int32_t n, int32_
     return launch_p2p<double>(s, (const double*)pos4, n, leaf_size, num_leaves, near_off, near_src, eps2, G,
                               out_index, (double*)acc, (double*)potp, comp, accumulate != 0);
   return fail(JFM_ERR_ARG, "jfm_p2p: bad dtype");
+}
+
+int32_t jfm_p2p_items(void* stream, int32_t dtype, const void* pos4, int32_t n, int32_t num_leaves, int32_t leaf_size,
+                      int32_t num_items, const int32_t* item_leaf, const int32_t* item_s0, const int32_t* item_s1,
+                      const int32_t* item_part, int32_t num_part, void* partials, const int32_t* near_off,
+                      const int32_t* near_src, double eps2, double G, const int32_t* out_index, void* acc, void* pot,
+                      int32_t flags, int32_t accumulate) {
+  if (n < 1 || leaf_size < 1 || leaf_size > 256 || num_leaves != (n + leaf_size - 1) / leaf_size || num_items < 1 ||
+      num_part < 0)
+    return fail(JFM_ERR_ARG, "jfm_p2p_items: inconsistent sizes");
+  if (!pos4 || !near_off || !near_src || !acc || !item_leaf || !item_s0 || !item_s1 || !item_part ||
+      (num_part > 0 && !partials))
+    return fail(JFM_ERR_ARG, "jfm_p2p_items: NULL buffer");
+  if (!(eps2 >= 0.0)) return fail(JFM_ERR_ARG, "jfm_p2p_items: eps2 must be >= 0");
+  if ((flags & JFM_EVAL_POTENTIAL) && !pot) return fail(JFM_ERR_ARG, "jfm_p2p_items: pot is NULL");
+  hipStream_t s = as_stream(stream);
+  const bool comp = (flags & JFM_EVAL_COMPENSATED) != 0;
+  void* potp = (flags & JFM_EVAL_POTENTIAL) ? pot : nullptr;
+  if (dtype == JFM_F32)
+    return launch_p2p<float>(s, (const float*)pos4, n, leaf_size, num_items, near_off, near_src,
+                             static_cast<float>(eps2), static_cast<float>(G), out_index, (float*)acc, (float*)potp,
+                             comp, accumulate != 0,
+                             P2PItems<float>{item_leaf, item_s0, item_s1, item_part, (float*)partials, num_part});
+  if (dtype == JFM_F64)
+    return launch_p2p<double>(s, (const double*)pos4, n, leaf_size, num_items, near_off, near_src, eps2, G,
+                              out_index, (double*)acc, (double*)potp, comp, accumulate != 0,
+                              P2PItems<double>{item_leaf, item_s0, item_s1, item_part, (double*)partials, num_part});
+  return fail(JFM_ERR_ARG, "jfm_p2p_items: bad dtype");
+}
+
+int32_t jfm_p2p_reduce(void* stream, int32_t dtype, const void* pos4, int32_t n, int32_t leaf_size, int32_t num_split,
+                       const int32_t* split_leaf, const int32_t* split_p0, const int32_t* split_p1,
+                       const void* partials, double eps2, double G, const int32_t* out_index, void* acc, void* pot,
+                       int32_t flags, int32_t accumulate) {
+  if (n < 1 || leaf_size < 1 || leaf_size > 256 || num_split < 0) return fail(JFM_ERR_ARG, "jfm_p2p_reduce: sizes");
+  if (num_split == 0) return JFM_OK;
+  if (!pos4 || !split_leaf || !split_p0 || !split_p1 || !partials || !acc)
+    return fail(JFM_ERR_ARG, "jfm_p2p_reduce: NULL buffer");
+  const bool want_pot = (flags & JFM_EVAL_POTENTIAL) != 0;
+  if (want_pot && !pot) return fail(JFM_ERR_ARG, "jfm_p2p_reduce: pot is NULL");
+  hipStream_t s = as_stream(stream);
+  const unsigned blocks = static_cast<unsigned>((int64_t(num_split) * leaf_size + kThreads - 1) / kThreads);
+#define JFM_P2P_REDUCE(T, POT)                                                                                  \
+  p2p_reduce_kernel<T, POT><<<blocks, kThreads, 0, s>>>((const T*)pos4, n, leaf_size, num_split, split_leaf,     \
+                                                        split_p0, split_p1, (const T*)partials,                \
+                                                        static_cast<T>(eps2), static_cast<T>(G), out_index,    \
+                                                        (T*)acc, (T*)pot, accumulate)
+  if (dtype == JFM_F32) {
+    if (want_pot) JFM_P2P_REDUCE(float, true); else JFM_P2P_REDUCE(float, false);
+  } else if (dtype == JFM_F64) {
+    if (want_pot) JFM_P2P_REDUCE(double, true); else JFM_P2P_REDUCE(double, false);
+  } else {
+    return fail(JFM_ERR_ARG, "jfm_p2p_reduce: bad dtype");
+  }
+#undef JFM_P2P_REDUCE
+  return check_launch("jfm_p2p_reduce");
 }
 
 int32_t jfm_evaluate(void* stream, int32_t dtype, int32_t p, const void* pos4, int32_t n, int32_t num_leaves,

@@ -26,6 +26,7 @@ import torch
 
 from . import _native as nat
 from .flatlist import build_flat_list, launch_flat, pair_targets, prefers_flat
+from .nearsplit import build_p2p_items
 from .dtopology import build_topology_device
 from .topology import Topology, build_topology
 
@@ -101,7 +102,10 @@ class FastMultipoleMethod:
     largest gear otherwise); ``max_order`` must be the largest gear.  Gears
     are fixed in ``prepare_state``.  ``tree_build`` picks where ``prepare_state``
     builds the topology: ``"device"`` (HIP kernels, default) or ``"host"``
-    (C++ threads); both give the same arrays bit for bit.
+    (C++ threads); both give the same arrays bit for bit.  ``split_near``
+    (default on) cuts long near lists into P2P work items of at most 2048
+    source slots (``nearsplit``), so a few leaves with huge near lists do not
+    run alone at the end of the P2P launch.
     """
 
     def __init__(
@@ -123,6 +127,7 @@ class FastMultipoleMethod:
         gear_tolerance: float = -2.5,
         num_threads: int = 0,
         tree_build: str = "device",
+        split_near: bool = True,
         **legacy_kwargs: Any,
     ):
         preset_norm = str(getattr(preset, "value", preset)).strip().lower()
@@ -174,6 +179,7 @@ class FastMultipoleMethod:
         self.gear_tolerance = float(gear_tolerance)
         self.num_threads = int(num_threads)
         self.tree_build = tree_build
+        self.split_near = bool(split_near)
         self.device = _require_gpu()
         self._last_state: Optional[FMMPreparedState] = None
 
@@ -257,6 +263,8 @@ class FastMultipoleMethod:
         if prefers_flat(topo.num_far_pairs, int(topo.m2l_targets.size)):
             state.extra["flat"] = build_flat_list(pair_targets(state), state.far_src[: topo.num_far_pairs], p, dtype,
                                                   dev)
+        if self.split_near:
+            state.extra["p2p_items"] = build_p2p_items(topo.near_off, topo.leaf_size, dtype, dev)
         if self.adaptive_order:
             self._prepare_gears(state, xd, md)
         self.refresh_prepared_state(state, xd, md)
@@ -400,11 +408,56 @@ class FastMultipoleMethod:
                 flags |= nat.EVAL_COMPENSATED
         if pot is not None:
             flags |= nat.EVAL_POTENTIAL
-        nat.call("jfm_evaluate", nat.current_stream(), nat.dtype_code(state.dtype), state.order,
-                 state.pos4.data_ptr(), topo.num_particles, topo.num_leaves, topo.leaf_size, topo.num_internal,
-                 state.near_off.data_ptr(), state.near_src.data_ptr(), state.locals.data_ptr(),
-                 state.centers4.data_ptr(), state.softening ** 2, state.G, state.perm.data_ptr(), acc.data_ptr(),
-                 pot.data_ptr() if pot is not None else None, flags)
+        s = nat.current_stream()
+        code = nat.dtype_code(state.dtype)
+        potp = pot.data_ptr() if pot is not None else None
+        items = state.extra.get("p2p_items")
+        if items is None:
+            nat.call("jfm_evaluate", s, code, state.order, state.pos4.data_ptr(), topo.num_particles,
+                     topo.num_leaves, topo.leaf_size, topo.num_internal, state.near_off.data_ptr(),
+                     state.near_src.data_ptr(), state.locals.data_ptr(), state.centers4.data_ptr(),
+                     state.softening ** 2, state.G, state.perm.data_ptr(), acc.data_ptr(), potp, flags)
+            return
+        far = bool(flags & nat.EVAL_FAR)
+        if far:
+            nat.call("jfm_l2p", s, code, state.order, state.pos4.data_ptr(), topo.num_particles, topo.leaf_size,
+                     topo.num_internal, state.locals.data_ptr(), state.centers4.data_ptr(), state.G,
+                     state.perm.data_ptr(), acc.data_ptr(), potp)
+        if flags & nat.EVAL_NEAR:
+            self._p2p_items(state, items, acc, pot, flags, accumulate=far)
+
+    @staticmethod
+    def _p2p_items(state: FMMPreparedState, items, acc: torch.Tensor, pot: Optional[torch.Tensor], flags: int,
+                   accumulate: bool) -> None:
+        """P2P over the state's work items, then the partial sums of split leaves."""
+        topo = state.topology
+        s = nat.current_stream()
+        code = nat.dtype_code(state.dtype)
+        potp = pot.data_ptr() if pot is not None else None
+        nat.call("jfm_p2p_items", s, code, state.pos4.data_ptr(), topo.num_particles, topo.num_leaves,
+                 topo.leaf_size, items.num_items, items.leaf.data_ptr(), items.s0.data_ptr(), items.s1.data_ptr(),
+                 items.part.data_ptr(), items.num_part, items.partials.data_ptr(), state.near_off.data_ptr(),
+                 state.near_src.data_ptr(), state.softening ** 2, state.G, state.perm.data_ptr(), acc.data_ptr(),
+                 potp, flags, 1 if accumulate else 0)
+        nat.call("jfm_p2p_reduce", s, code, state.pos4.data_ptr(), topo.num_particles, topo.leaf_size,
+                 items.num_split, items.split_leaf.data_ptr(), items.split_p0.data_ptr(), items.split_p1.data_ptr(),
+                 items.partials.data_ptr(), state.softening ** 2, state.G, state.perm.data_ptr(), acc.data_ptr(),
+                 potp, flags, 1 if accumulate else 0)
+
+    def _p2p_device(self, state: FMMPreparedState, acc: torch.Tensor, pot: Optional[torch.Tensor], flags: int,
+                    accumulate: bool) -> None:
+        """Near field only (as jfm_p2p), through the work items when the state has them."""
+        items = state.extra.get("p2p_items")
+        if pot is not None:
+            flags |= nat.EVAL_POTENTIAL
+        if items is not None:
+            self._p2p_items(state, items, acc, pot, flags, accumulate)
+            return
+        topo = state.topology
+        nat.call("jfm_p2p", nat.current_stream(), nat.dtype_code(state.dtype), state.pos4.data_ptr(),
+                 topo.num_particles, topo.num_leaves, topo.leaf_size, state.near_off.data_ptr(),
+                 state.near_src.data_ptr(), state.softening ** 2, state.G, state.perm.data_ptr(), acc.data_ptr(),
+                 pot.data_ptr() if pot is not None else None, flags, 1 if accumulate else 0)
 
     def evaluate_prepared_state(self, state: FMMPreparedState, *, target_indices=None,
                                 return_potential: bool = False, jit_traversal=None,

@@ -346,3 +346,37 @@ def test_benchmark_workload_matches_cpu_restatement(require_gpu):
     print(f"[gpu]   rel-L2 {rel:.3e}; per particle median {np.median(per):.3e} p99 {np.quantile(per, 0.99):.3e}",
           flush=True)
     assert rel < 1e-5
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_split_near_lists_match_whole_leaves(precision, require_gpu):
+    """P2P through work items (long near lists cut into slot ranges, partial
+    sums added in item order) equals the one-workgroup-per-leaf P2P: to
+    rounding in float64, to float32 summation-order noise in float32; with
+    potentials, with and without the far field."""
+    from jaccpot_amd import FastMultipoleMethod
+    from jaccpot_amd import _native as nat
+    from jaccpot_amd.nearsplit import build_p2p_items
+
+    print(f"[gpu] split near lists {precision}", flush=True)
+    x, m = ics.plummer(20000, 13)
+    if precision == "fp32":
+        x, m = x.astype(np.float32), m.astype(np.float32)
+    fmm = FastMultipoleMethod(precision=precision, theta=0.7, softening=1e-3, split_near=False)
+    state = fmm.prepare_state(x, m, leaf_size=8, max_order=4)
+    assert "p2p_items" not in state.extra or state.extra["p2p_items"] is None
+    whole, whole_pot = fmm.evaluate_prepared_state(state, return_potential=True)
+    near_whole = torch.empty((x.shape[0], 3), dtype=state.dtype, device="cuda")
+    fmm._evaluate_device(state, near_whole, None, flags=nat.EVAL_NEAR | nat.EVAL_COMPENSATED)
+    items = build_p2p_items(state.topology.near_off, 8, state.dtype, "cuda", chunk_slots=96)
+    print(f"[gpu]   {items.num_items} items, {items.num_split} split leaves", flush=True)
+    assert items.num_split > 100
+    state.extra["p2p_items"] = items
+    split, split_pot = fmm.evaluate_prepared_state(state, return_potential=True)
+    near_split = torch.empty_like(near_whole)
+    fmm._evaluate_device(state, near_split, None, flags=nat.EVAL_NEAR | nat.EVAL_COMPENSATED)
+    torch.cuda.synchronize()
+    tol = 1e-13 if precision == "fp64" else 2e-6
+    for a, b in ((split, whole), (split_pot, whole_pot), (near_split.cpu().numpy(), near_whole.cpu().numpy())):
+        rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+        assert rel < tol, rel
