@@ -256,40 +256,52 @@ __global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos
   }
 }
 
-// Leaves split over several items: target i of split leaf k adds the partials
-// of items p0[k] .. p1[k]-1 in order (Kahan), then applies what the single-item
-// path applies (self potential, G, output row).
+// Leaves split over several items: one workgroup per split leaf.  Target i
+// of the leaf has nsub = 256 / leaf_size lanes; lane j adds the partials of
+// items p0 + j, p0 + j + nsub, ... (compensated), and lane 0 then adds the
+// nsub lane sums in lane order: a fixed summation order for every run.  The
+// result gets what the single-item path applies (self potential, G, output row).
 template <typename T, bool POT>
 __global__ __launch_bounds__(kThreads) void p2p_reduce_kernel(const T* __restrict__ pos4, int32_t n, int32_t leaf_size,
-                                                              int32_t num_split, const int32_t* __restrict__ split_leaf,
+                                                              const int32_t* __restrict__ split_leaf,
                                                               const int32_t* __restrict__ p0,
                                                               const int32_t* __restrict__ p1,
                                                               const T* __restrict__ partials, T eps2, T G,
                                                               const int32_t* __restrict__ out_index,
                                                               T* __restrict__ acc, T* __restrict__ pot,
                                                               int32_t accumulate) {
-  const int64_t t = int64_t(blockIdx.x) * kThreads + threadIdx.x;
-  if (t >= int64_t(num_split) * leaf_size) return;
-  const int k = static_cast<int>(t / leaf_size), i = static_cast<int>(t % leaf_size);
-  const int first = split_leaf[k] * leaf_size;
-  if (first + i >= n) return;
+  __shared__ T red[4][kThreads];
+  const int k = blockIdx.x;
+  const int nsub = kThreads / leaf_size;
+  const int i = threadIdx.x % leaf_size, j = threadIdx.x / leaf_size;
   T s[4] = {0, 0, 0, 0}, c[4] = {0, 0, 0, 0};
-  for (int p = p0[k]; p < p1[k]; ++p) {
-    const T* src = partials + (static_cast<int64_t>(p) * leaf_size + i) * 4;
-    for (int q = 0; q < 4; ++q) kahan_add(s[q], c[q], src[q]);
+  if (j < nsub) {
+    const int a = p0[k], b = p1[k];
+    for (int p = a + j; p < b; p += nsub) {
+      const T* src = partials + (static_cast<int64_t>(p) * leaf_size + i) * 4;
+      for (int q = 0; q < 4; ++q) kahan_add(s[q], c[q], src[q]);
+    }
   }
-  T ap = s[3];
+  for (int q = 0; q < 4; ++q) red[q][threadIdx.x] = s[q] - c[q];
+  __syncthreads();
+  const int first = split_leaf[k] * leaf_size;
+  if (j != 0 || first + i >= n) return;
+  T t[4] = {0, 0, 0, 0}, e[4] = {0, 0, 0, 0};
+  for (int jj = 0; jj < nsub; ++jj)
+    for (int q = 0; q < 4; ++q) kahan_add(t[q], e[q], red[q][jj * leaf_size + i]);
+  for (int q = 0; q < 4; ++q) t[q] -= e[q];
+  T ap = t[3];
   if (POT && eps2 > T(0)) ap -= pos4[4 * static_cast<int64_t>(first + i) + 3] * fast_rsqrt(eps2);
   const int64_t row = out_index ? out_index[first + i] : first + i;
   if (accumulate) {
-    acc[3 * row + 0] += G * s[0];
-    acc[3 * row + 1] += G * s[1];
-    acc[3 * row + 2] += G * s[2];
+    acc[3 * row + 0] += G * t[0];
+    acc[3 * row + 1] += G * t[1];
+    acc[3 * row + 2] += G * t[2];
     if (POT) pot[row] += -G * ap;
   } else {
-    acc[3 * row + 0] = G * s[0];
-    acc[3 * row + 1] = G * s[1];
-    acc[3 * row + 2] = G * s[2];
+    acc[3 * row + 0] = G * t[0];
+    acc[3 * row + 1] = G * t[1];
+    acc[3 * row + 2] = G * t[2];
     if (POT) pot[row] = -G * ap;
   }
 }
@@ -415,12 +427,11 @@ int32_t jfm_p2p_reduce(void* stream, int32_t dtype, const void* pos4, int32_t n,
   const bool want_pot = (flags & JFM_EVAL_POTENTIAL) != 0;
   if (want_pot && !pot) return fail(JFM_ERR_ARG, "jfm_p2p_reduce: pot is NULL");
   hipStream_t s = as_stream(stream);
-  const unsigned blocks = static_cast<unsigned>((int64_t(num_split) * leaf_size + kThreads - 1) / kThreads);
 #define JFM_P2P_REDUCE(T, POT)                                                                                  \
-  p2p_reduce_kernel<T, POT><<<blocks, kThreads, 0, s>>>((const T*)pos4, n, leaf_size, num_split, split_leaf,     \
-                                                        split_p0, split_p1, (const T*)partials,                \
-                                                        static_cast<T>(eps2), static_cast<T>(G), out_index,    \
-                                                        (T*)acc, (T*)pot, accumulate)
+  p2p_reduce_kernel<T, POT><<<num_split, kThreads, 0, s>>>((const T*)pos4, n, leaf_size, split_leaf, split_p0,   \
+                                                           split_p1, (const T*)partials, static_cast<T>(eps2), \
+                                                           static_cast<T>(G), out_index, (T*)acc, (T*)pot,     \
+                                                           accumulate)
   if (dtype == JFM_F32) {
     if (want_pot) JFM_P2P_REDUCE(float, true); else JFM_P2P_REDUCE(float, false);
   } else if (dtype == JFM_F64) {
