@@ -17,7 +17,9 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
-CHUNK_SLOTS = 2048  # two float32 source tiles per item
+# Slots per item: measured on MI355X (Plummer N=4M, leaf 8 / 64), 512: 8.2 / 69.4 ms,
+# 2048: 4.17 / 64.3 ms, 8192: 3.96 / 63.8 ms per P2P launch.
+CHUNK_SLOTS = 8192
 
 
 @dataclass

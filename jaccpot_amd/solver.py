@@ -103,7 +103,7 @@ class FastMultipoleMethod:
     are fixed in ``prepare_state``.  ``tree_build`` picks where ``prepare_state``
     builds the topology: ``"device"`` (HIP kernels, default) or ``"host"``
     (C++ threads); both give the same arrays bit for bit.  ``split_near``
-    (default on) cuts long near lists into P2P work items of at most 2048
+    (default on) cuts long near lists into P2P work items of at most 8192
     source slots (``nearsplit``), so a few leaves with huge near lists do not
     run alone at the end of the P2P launch.
     """
