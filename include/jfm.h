@@ -343,6 +343,17 @@ int32_t jfm_evaluate(void* stream, int32_t dtype, int32_t p, const void* pos4, i
 int32_t jfm_p2p_adjoint(void* stream, int32_t dtype, const void* pos4, const void* cot4, int32_t n,
                         int32_t num_leaves, int32_t leaf_size, const int32_t* near_off, const int32_t* near_src,
                         double eps2, double G, void* grad4);
+/* jfm_p2p_adjoint over the work items of jfm_p2p_items (same item arrays):
+ * single-item leaves store their grad4 rows, split leaves leave partials that
+ * jfm_p2p_adjoint_reduce adds in item order and stores. */
+int32_t jfm_p2p_adjoint_items(void* stream, int32_t dtype, const void* pos4, const void* cot4, int32_t n,
+                              int32_t num_leaves, int32_t leaf_size, int32_t num_items, const int32_t* item_leaf,
+                              const int32_t* item_s0, const int32_t* item_s1, const int32_t* item_part,
+                              int32_t num_part, void* partials, const int32_t* near_off, const int32_t* near_src,
+                              double eps2, double G, void* grad4);
+int32_t jfm_p2p_adjoint_reduce(void* stream, int32_t dtype, const void* cot4, int32_t n, int32_t leaf_size,
+                               int32_t num_split, const int32_t* split_leaf, const int32_t* split_p0,
+                               const int32_t* split_p1, const void* partials, double eps2, double G, void* grad4);
 int32_t jfm_leaf_adjoint_a(void* stream, int32_t dtype, int32_t p, const void* pos4, const void* cot4, int32_t n,
                            int32_t leaf_size, int32_t num_internal, const void* locals, const void* centers4,
                            double G, void* grad4, void* lbar_flip);

@@ -96,6 +96,11 @@ _SIGNATURES = {
                                c_void_p, c_void_p, c_void_p, c_double, c_double, c_void_p, c_void_p, c_void_p,
                                c_int32]),
     "jfm_direct_scratch_doubles": (c_int64, [c_int32]),
+    "jfm_p2p_adjoint_items": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_int32, c_int32, c_int32,
+                                        c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p,
+                                        c_double, c_double, c_void_p]),
+    "jfm_p2p_adjoint_reduce": (c_int32, [c_void_p, c_int32, c_void_p, c_int32, c_int32, c_int32, c_void_p, c_void_p,
+                                         c_void_p, c_void_p, c_double, c_double, c_void_p]),
     "jfm_p2p_adjoint": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_int32, c_int32, c_void_p,
                                   c_void_p, c_double, c_double, c_void_p]),
     "jfm_leaf_adjoint_a": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_int32, c_int32, c_int32,

@@ -66,8 +66,19 @@ def fmm_vjp(fmm, state, positions: torch.Tensor, masses: torch.Tensor, acc_bar: 
     nat.call("jfm_gather_sorted", s, code, code, ab.data_ptr(), pb.data_ptr(), state.perm.data_ptr(), n,
              cot4.data_ptr())
     grad4 = torch.empty((n, 4), dtype=dt, device=dev)
-    nat.call("jfm_p2p_adjoint", s, code, state.pos4.data_ptr(), cot4.data_ptr(), n, topo.num_leaves, topo.leaf_size,
-             state.near_off.data_ptr(), state.near_src.data_ptr(), state.softening ** 2, state.G, grad4.data_ptr())
+    items = state.extra.get("p2p_items")
+    if items is None:
+        nat.call("jfm_p2p_adjoint", s, code, state.pos4.data_ptr(), cot4.data_ptr(), n, topo.num_leaves,
+                 topo.leaf_size, state.near_off.data_ptr(), state.near_src.data_ptr(), state.softening ** 2, state.G,
+                 grad4.data_ptr())
+    else:  # long near lists cut into work items, as in the forward P2P
+        nat.call("jfm_p2p_adjoint_items", s, code, state.pos4.data_ptr(), cot4.data_ptr(), n, topo.num_leaves,
+                 topo.leaf_size, items.num_items, items.leaf.data_ptr(), items.s0.data_ptr(), items.s1.data_ptr(),
+                 items.part.data_ptr(), items.num_part, items.partials.data_ptr(), state.near_off.data_ptr(),
+                 state.near_src.data_ptr(), state.softening ** 2, state.G, grad4.data_ptr())
+        nat.call("jfm_p2p_adjoint_reduce", s, code, cot4.data_ptr(), n, topo.leaf_size, items.num_split,
+                 items.split_leaf.data_ptr(), items.split_p0.data_ptr(), items.split_p1.data_ptr(),
+                 items.partials.data_ptr(), state.softening ** 2, state.G, grad4.data_ptr())
     lbar = torch.zeros_like(state.locals)
     nat.call("jfm_leaf_adjoint_a", s, code, state.order, state.pos4.data_ptr(), cot4.data_ptr(), n, topo.leaf_size,
              topo.num_internal, state.locals.data_ptr(), state.centers4.data_ptr(), state.G, grad4.data_ptr(),

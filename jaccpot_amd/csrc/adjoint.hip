@@ -154,18 +154,34 @@ __device__ __forceinline__ void weighted_harmonics(T dx, T dy, T dz, T w, T bx, 
 // ------------------------------------------------------------------ P2P
 // xbar_i += G [ (I s3 - 3 d d^T s5)(m_i abar_j - m_j abar_i) - d s3 (m_j phib_i + m_i phib_j) ]
 // mbar_i += G [ -d s3 . abar_j - s phib_j ],  d = x_j - x_i, over self (j != i) + near leaves.
+//
+// Work items as in the forward P2P: item k covers slots [s0, s1) of leaf
+// `leaf`; part < 0 -> the item stores the leaf's rows of grad4, else its
+// unscaled partial sums go to partials[(part * leaf_size + i) * 4 + c] and
+// p2p_adjoint_reduce_kernel adds them in item order.  leaf == NULL: one item
+// per leaf.
+template <typename T>
+struct AdjItems {
+  const int32_t* leaf;
+  const int32_t* s0;
+  const int32_t* s1;
+  const int32_t* part;
+  T* partials;
+  int32_t num_part;
+};
+
 template <typename T, int WT, bool SAFE>
 __global__ __launch_bounds__(kThreads) void p2p_adjoint_kernel(const T* __restrict__ pos4, const T* __restrict__ cot4,
                                                                int32_t n, int32_t leaf_size,
                                                                const int32_t* __restrict__ near_off,
                                                                const int32_t* __restrict__ near_src, T eps2, T G,
-                                                               T* __restrict__ grad4) {
+                                                               T* __restrict__ grad4, AdjItems<T> items) {
   constexpr int NPART = kThreads / WT;
   constexpr int TILE = sizeof(T) == 4 ? 512 : 256;
   constexpr int LOADS = TILE / kThreads;
   static_assert(2 * kThreads <= 2 * TILE, "reduction buffer");
   __shared__ V4<T> tx[TILE], tc[TILE];
-  const int leaf = blockIdx.x;
+  const int leaf = items.leaf ? items.leaf[blockIdx.x] : blockIdx.x;
   const int i = threadIdx.x % WT, part = threadIdx.x / WT;
   const int first = leaf * leaf_size;
   const int cnt = min(leaf_size, n - first);
@@ -173,9 +189,11 @@ __global__ __launch_bounds__(kThreads) void p2p_adjoint_kernel(const T* __restri
   const V4<T> xi = load4(pos4, me);
   const V4<T> ci = load4(cot4, me);
   const int nb = near_off[leaf];
-  const int total = (1 + near_off[leaf + 1] - nb) * leaf_size;
+  const int start = items.leaf ? items.s0[blockIdx.x] : 0;
+  const int total = items.leaf ? items.s1[blockIdx.x] : (1 + near_off[leaf + 1] - nb) * leaf_size;
+  const int part_out = items.leaf ? items.part[blockIdx.x] : -1;
   T gx = 0, gy = 0, gz = 0, gm = 0;
-  for (int base = 0; base < total; base += TILE) {
+  for (int base = start; base < total; base += TILE) {
     __syncthreads();
     for (int l = 0; l < LOADS; ++l) {
       const int slot = base + l * kThreads + threadIdx.x;
@@ -225,9 +243,56 @@ __global__ __launch_bounds__(kThreads) void p2p_adjoint_kernel(const T* __restri
       az += v.z;
       am += v.w;
     }
+    if (part_out >= 0) {
+      if (part_out < items.num_part)
+        store4(items.partials, static_cast<int64_t>(part_out) * leaf_size + i, V4<T>{ax, ay, az, am});
+      return;
+    }
     if (eps2 > T(0)) am += ci.w / dev_sqrt(eps2);  // the self term is not in the forward potential
     store4(grad4, first + i, V4<T>{G * ax, G * ay, G * az, G * am});
   }
+}
+
+template <typename T>
+__device__ __forceinline__ void kahan_add(T& s, T& c, T v) {
+  const T y = v - c;
+  const T t = s + y;
+  c = (t - s) - y;
+  s = t;
+}
+
+// One workgroup per split leaf: 256 / leaf_size lanes per target add strided
+// item partials (compensated), lane 0 adds the lane sums in lane order.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void p2p_adjoint_reduce_kernel(const T* __restrict__ cot4, int32_t n,
+                                                                      int32_t leaf_size,
+                                                                      const int32_t* __restrict__ split_leaf,
+                                                                      const int32_t* __restrict__ p0,
+                                                                      const int32_t* __restrict__ p1,
+                                                                      const T* __restrict__ partials, T eps2, T G,
+                                                                      T* __restrict__ grad4) {
+  __shared__ T red[4][kThreads];
+  const int k = blockIdx.x;
+  const int nsub = kThreads / leaf_size;
+  const int i = threadIdx.x % leaf_size, j = threadIdx.x / leaf_size;
+  T s[4] = {0, 0, 0, 0}, c[4] = {0, 0, 0, 0};
+  if (j < nsub) {
+    const int a = p0[k], b = p1[k];
+    for (int p = a + j; p < b; p += nsub) {
+      const T* src = partials + (static_cast<int64_t>(p) * leaf_size + i) * 4;
+      for (int q = 0; q < 4; ++q) kahan_add(s[q], c[q], src[q]);
+    }
+  }
+  for (int q = 0; q < 4; ++q) red[q][threadIdx.x] = s[q] - c[q];
+  __syncthreads();
+  const int first = split_leaf[k] * leaf_size;
+  if (j != 0 || first + i >= n) return;
+  T t[4] = {0, 0, 0, 0}, e[4] = {0, 0, 0, 0};
+  for (int jj = 0; jj < nsub; ++jj)
+    for (int q = 0; q < 4; ++q) kahan_add(t[q], e[q], red[q][jj * leaf_size + i]);
+  for (int q = 0; q < 4; ++q) t[q] -= e[q];
+  if (eps2 > T(0)) t[3] += cot4[4 * static_cast<int64_t>(first + i) + 3] / dev_sqrt(eps2);
+  store4(grad4, first + i, V4<T>{G * t[0], G * t[1], G * t[2], G * t[3]});
 }
 
 // ------------------------------------------------------------- leaf adjoint A
@@ -460,16 +525,17 @@ __global__ __launch_bounds__(kThreads) void leaf_adjoint_b_kernel(const T* __res
 
 template <typename T>
 int32_t launch_p2p_adj(hipStream_t s, const T* pos4, const T* cot4, int32_t n, int32_t leaf_size,
-                       int32_t num_leaves, const int32_t* near_off, const int32_t* near_src, T eps2, T G, T* grad4) {
+                       int32_t num_blocks, const int32_t* near_off, const int32_t* near_src, T eps2, T G, T* grad4,
+                       AdjItems<T> items = AdjItems<T>{}) {
   const bool safe = !(eps2 > T(0));
 #define JFM_ADJ(WT)                                                                                            \
   {                                                                                                            \
     if (safe)                                                                                                  \
-      p2p_adjoint_kernel<T, WT, true><<<num_leaves, kThreads, 0, s>>>(pos4, cot4, n, leaf_size, near_off,      \
-                                                                      near_src, eps2, G, grad4);               \
+      p2p_adjoint_kernel<T, WT, true><<<num_blocks, kThreads, 0, s>>>(pos4, cot4, n, leaf_size, near_off,      \
+                                                                      near_src, eps2, G, grad4, items);        \
     else                                                                                                       \
-      p2p_adjoint_kernel<T, WT, false><<<num_leaves, kThreads, 0, s>>>(pos4, cot4, n, leaf_size, near_off,     \
-                                                                       near_src, eps2, G, grad4);              \
+      p2p_adjoint_kernel<T, WT, false><<<num_blocks, kThreads, 0, s>>>(pos4, cot4, n, leaf_size, near_off,     \
+                                                                       near_src, eps2, G, grad4, items);       \
     return check_launch("jfm_p2p_adjoint");                                                                    \
   }
   if (leaf_size <= 16) JFM_ADJ(16)
@@ -503,6 +569,52 @@ int32_t jfm_p2p_adjoint(void* stream, int32_t dtype, const void* pos4, const voi
     return launch_p2p_adj<double>(s, (const double*)pos4, (const double*)cot4, n, leaf_size, num_leaves, near_off,
                                   near_src, eps2, G, (double*)grad4);
   return fail(JFM_ERR_ARG, "jfm_p2p_adjoint: bad dtype");
+}
+
+int32_t jfm_p2p_adjoint_items(void* stream, int32_t dtype, const void* pos4, const void* cot4, int32_t n,
+                              int32_t num_leaves, int32_t leaf_size, int32_t num_items, const int32_t* item_leaf,
+                              const int32_t* item_s0, const int32_t* item_s1, const int32_t* item_part,
+                              int32_t num_part, void* partials, const int32_t* near_off, const int32_t* near_src,
+                              double eps2, double G, void* grad4) {
+  if (n < 1 || leaf_size < 1 || leaf_size > 256 || num_leaves != (n + leaf_size - 1) / leaf_size || num_items < 1 ||
+      num_part < 0)
+    return fail(JFM_ERR_ARG, "jfm_p2p_adjoint_items: inconsistent sizes");
+  if (!pos4 || !cot4 || !near_off || !near_src || !grad4 || !item_leaf || !item_s0 || !item_s1 || !item_part ||
+      (num_part > 0 && !partials))
+    return fail(JFM_ERR_ARG, "jfm_p2p_adjoint_items: NULL buffer");
+  hipStream_t s = as_stream(stream);
+  if (dtype == JFM_F32)
+    return launch_p2p_adj<float>(s, (const float*)pos4, (const float*)cot4, n, leaf_size, num_items, near_off,
+                                 near_src, static_cast<float>(eps2), static_cast<float>(G), (float*)grad4,
+                                 AdjItems<float>{item_leaf, item_s0, item_s1, item_part, (float*)partials, num_part});
+  if (dtype == JFM_F64)
+    return launch_p2p_adj<double>(s, (const double*)pos4, (const double*)cot4, n, leaf_size, num_items, near_off,
+                                  near_src, eps2, G, (double*)grad4,
+                                  AdjItems<double>{item_leaf, item_s0, item_s1, item_part, (double*)partials,
+                                                   num_part});
+  return fail(JFM_ERR_ARG, "jfm_p2p_adjoint_items: bad dtype");
+}
+
+int32_t jfm_p2p_adjoint_reduce(void* stream, int32_t dtype, const void* cot4, int32_t n, int32_t leaf_size,
+                               int32_t num_split, const int32_t* split_leaf, const int32_t* split_p0,
+                               const int32_t* split_p1, const void* partials, double eps2, double G, void* grad4) {
+  if (n < 1 || leaf_size < 1 || leaf_size > 256 || num_split < 0)
+    return fail(JFM_ERR_ARG, "jfm_p2p_adjoint_reduce: sizes");
+  if (num_split == 0) return JFM_OK;
+  if (!cot4 || !split_leaf || !split_p0 || !split_p1 || !partials || !grad4)
+    return fail(JFM_ERR_ARG, "jfm_p2p_adjoint_reduce: NULL buffer");
+  hipStream_t s = as_stream(stream);
+  if (dtype == JFM_F32)
+    p2p_adjoint_reduce_kernel<float><<<num_split, kThreads, 0, s>>>(
+        (const float*)cot4, n, leaf_size, split_leaf, split_p0, split_p1, (const float*)partials,
+        static_cast<float>(eps2), static_cast<float>(G), (float*)grad4);
+  else if (dtype == JFM_F64)
+    p2p_adjoint_reduce_kernel<double><<<num_split, kThreads, 0, s>>>(
+        (const double*)cot4, n, leaf_size, split_leaf, split_p0, split_p1, (const double*)partials, eps2, G,
+        (double*)grad4);
+  else
+    return fail(JFM_ERR_ARG, "jfm_p2p_adjoint_reduce: bad dtype");
+  return check_launch("jfm_p2p_adjoint_reduce");
 }
 
 int32_t jfm_leaf_adjoint_a(void* stream, int32_t dtype, int32_t p, const void* pos4, const void* cot4, int32_t n,
