@@ -136,6 +136,7 @@ def build_topology_device(x: torch.Tensor, m: torch.Tensor, *, leaf_size: int, t
     nat.call("jfm_tree_morton", s, x.data_ptr(), n, b6.data_ptr(), codes.data_ptr(), index.data_ptr())
     sorted_codes, perm = _sort_u64(codes, 63, index)
     del codes, index
+    bounds_used = b6
 
     # ---- radix tree, levels ---------------------------------------------------------------
     parent = torch.empty(nodes, dtype=torch.int32, device=dev)
@@ -228,7 +229,8 @@ def build_topology_device(x: torch.Tensor, m: torch.Tensor, *, leaf_size: int, t
     torch.cuda.synchronize(dev)
     device = dict(perm=perm, parent=parent, left=left, right=right, node_range=node_range, depth=depth,
                   center=center, radius=radius, mass=mass, far_off=far_off, far_src=far_src, near_off=near_off,
-                  near_src=near_src, m2m_nodes=m2m_nodes, l2l_nodes=l2l_nodes)
+                  near_src=near_src, m2m_nodes=m2m_nodes, l2l_nodes=l2l_nodes, sorted_codes=sorted_codes,
+                  bounds=bounds_used)
     t1 = time.perf_counter()
 
     def h(t):
@@ -251,3 +253,28 @@ def build_topology_device(x: torch.Tensor, m: torch.Tensor, *, leaf_size: int, t
     topo.extra["device"] = device
     topo.extra["device_build_seconds"] = t1 - t0
     return topo
+
+
+def sorted_morton(x: torch.Tensor, bounds=None):
+    """(sorted Morton codes, perm) of float64 device positions x [N,3]: the first
+    steps of ``build_topology_device`` (the topology reuse check)."""
+    n = int(x.shape[0])
+    dev = x.device
+    s = nat.current_stream()
+    x = x.contiguous()
+    flags = torch.zeros(1, dtype=torch.int32, device=dev)
+    bscratch = torch.empty(int(nat.call("jfm_tree_bounds_scratch_doubles")), dtype=torch.float64, device=dev)
+    if bounds is not None:
+        lo, hi = bounds
+        b6 = torch.as_tensor(np.concatenate([np.asarray(lo, np.float64), np.asarray(hi, np.float64)]), device=dev)
+    else:
+        b6 = torch.empty(6, dtype=torch.float64, device=dev)
+    nat.call("jfm_tree_bounds", s, x.data_ptr(), n, 1 if bounds is not None else 0, bscratch.data_ptr(),
+             b6.data_ptr(), flags.data_ptr())
+    codes = torch.empty(n, dtype=torch.int64, device=dev)
+    index = torch.empty(n, dtype=torch.int32, device=dev)
+    nat.call("jfm_tree_morton", s, x.data_ptr(), n, b6.data_ptr(), codes.data_ptr(), index.data_ptr())
+    sc, perm = _sort_u64(codes, 63, index)
+    if int(flags.item()) & 1:
+        raise ValueError("positions must be finite")
+    return sc, perm

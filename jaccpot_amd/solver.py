@@ -18,6 +18,7 @@ missing the constructor raises instead of falling back.
 
 from __future__ import annotations
 
+import weakref
 from dataclasses import dataclass, field
 from typing import Any, Optional, Sequence
 
@@ -105,7 +106,12 @@ class FastMultipoleMethod:
     (C++ threads); both give the same arrays bit for bit.  ``split_near``
     (default on) cuts long near lists into P2P work items of at most 8192
     source slots (``nearsplit``), so a few leaves with huge near lists do not
-    run alone at the end of the P2P launch.
+    run alone at the end of the P2P launch.  ``reuse_topology`` with
+    ``rebuild_every`` (solver.py:592-597) keeps the tree and the far/near lists
+    of the last ``prepare_state`` for up to ``rebuild_every - 1`` further calls
+    whose sorted Morton codes are unchanged (runtime/fmm_prepare.py:797-802):
+    the particle order, centres, expansions and locals are recomputed, the
+    interaction lists are not.
     """
 
     def __init__(
@@ -128,6 +134,8 @@ class FastMultipoleMethod:
         num_threads: int = 0,
         tree_build: str = "device",
         split_near: bool = True,
+        reuse_topology: bool = False,
+        rebuild_every: int = 1,
         **legacy_kwargs: Any,
     ):
         preset_norm = str(getattr(preset, "value", preset)).strip().lower()
@@ -153,6 +161,8 @@ class FastMultipoleMethod:
             raise ValueError("softening must be >= 0")
         if tree_build not in ("device", "host"):
             raise ValueError("tree_build must be 'device' or 'host'")
+        if int(rebuild_every) <= 0:
+            raise ValueError("rebuild_every must be positive")
         dtype = None
         if precision is not None:
             p = str(precision).strip().lower()
@@ -180,6 +190,10 @@ class FastMultipoleMethod:
         self.num_threads = int(num_threads)
         self.tree_build = tree_build
         self.split_near = bool(split_near)
+        self.reuse_topology = bool(reuse_topology)
+        self.rebuild_every = int(rebuild_every)
+        self._reuse: Optional[dict] = None
+        self._last_inputs: Optional[tuple] = None
         self.device = _require_gpu()
         self._last_state: Optional[FMMPreparedState] = None
 
@@ -228,6 +242,11 @@ class FastMultipoleMethod:
         dev = self.device
         xd = torch.as_tensor(x_host, device=dev)
         md = torch.as_tensor(m_host, device=dev)
+        key = (x_host.shape[0], int(leaf_size), p, th, dtype, None if bounds is None else
+               tuple(np.concatenate([np.asarray(bounds[0], float), np.asarray(bounds[1], float)]).tolist()))
+        reused = self._try_reuse(key, xd, md, bounds, kind, in_dtype, in_dev)
+        if reused is not None:
+            return reused
         if self.tree_build == "device":
             topo = build_topology_device(xd, md, leaf_size=int(leaf_size), theta=th, bounds=bounds)
         else:
@@ -268,6 +287,44 @@ class FastMultipoleMethod:
         if self.adaptive_order:
             self._prepare_gears(state, xd, md)
         self.refresh_prepared_state(state, xd, md)
+        self._last_state = state
+        if self.reuse_topology:
+            codes = on_dev.get("sorted_codes")
+            if codes is None:
+                from .dtopology import sorted_morton
+
+                codes, _ = sorted_morton(xd, bounds)
+            self._reuse = {"key": key, "codes": codes, "state": state, "count": 0}
+        return state
+
+    def _try_reuse(self, key, xd, md, bounds, kind, in_dtype, in_dev) -> Optional[FMMPreparedState]:
+        """A state on the cached tree and lists when the new sorted Morton codes
+        equal the cached ones and the reuse budget allows (fmm_prepare.py:797-802)."""
+        import copy
+        import dataclasses
+
+        from .dtopology import DeviceTopology, _Lazy, sorted_morton
+
+        r = self._reuse
+        if not self.reuse_topology or r is None or r["key"] != key or r["count"] >= self.rebuild_every - 1:
+            return None
+        codes, perm = sorted_morton(xd, bounds)
+        if not torch.equal(codes, r["codes"]):
+            return None
+        old = r["state"]
+        topo = copy.copy(old.topology)
+        topo.perm = _Lazy(perm) if isinstance(topo, DeviceTopology) else perm.cpu().numpy()
+        topo.extra = dict(topo.extra)
+        topo.extra["device"] = dict(topo.extra.get("device", {}), perm=perm)
+        state = dataclasses.replace(
+            old, topology=topo, perm=perm, input_kind=kind, input_dtype=in_dtype, input_device=in_dev,
+            pos4=torch.empty_like(old.pos4), centers4=torch.zeros_like(old.centers4),
+            multipoles=torch.zeros_like(old.multipoles), locals=torch.zeros_like(old.locals), extra=dict(old.extra))
+        if "mixed" in state.extra:
+            state.extra["mixed"] = self._mixed_schedule(topo, state.order, state.device)
+        self.refresh_prepared_state(state, xd, md)
+        r["count"] += 1
+        r["state"] = state
         self._last_state = state
         return state
 
@@ -511,9 +568,24 @@ class FastMultipoleMethod:
     def compute_accelerations(self, positions, masses, *, target_indices=None, bounds=None, leaf_size: int = 16,
                               max_order: int = 4, return_potential: bool = False, theta: Optional[float] = None,
                               reuse_prepared_state: bool = False, max_acc_derivative_order: int = 0):
-        """prepare_state + evaluate_prepared_state (solver.py:917)."""
-        state = self.prepare_state(positions, masses, bounds=bounds, leaf_size=leaf_size, max_order=max_order,
-                                   theta=theta)
+        """prepare_state + evaluate_prepared_state (solver.py:917).
+
+        ``reuse_prepared_state``: evaluate the last prepared state again when the
+        position and mass objects and the preparation arguments are the same
+        objects / values as in the call that prepared it (an identity check, like
+        the reference's: an array mutated in place is not noticed)."""
+        args = (int(leaf_size), int(max_order), theta)
+        last = self._last_inputs
+        if (reuse_prepared_state and bounds is None and self._last_state is not None and last is not None
+                and last[0]() is positions and last[1]() is masses and last[2] == args):
+            state = self._last_state
+        else:
+            state = self.prepare_state(positions, masses, bounds=bounds, leaf_size=leaf_size, max_order=max_order,
+                                       theta=theta)
+            try:
+                self._last_inputs = (weakref.ref(positions), weakref.ref(masses), args)
+            except TypeError:  # objects without weak references (lists) are never reused
+                self._last_inputs = None
         return self.evaluate_prepared_state(state, target_indices=target_indices, return_potential=return_potential,
                                             max_acc_derivative_order=max_acc_derivative_order)
 

@@ -380,3 +380,49 @@ def test_split_near_lists_match_whole_leaves(precision, require_gpu):
     for a, b in ((split, whole), (split_pot, whole_pot), (near_split.cpu().numpy(), near_whole.cpu().numpy())):
         rel = np.linalg.norm(a - b) / np.linalg.norm(b)
         assert rel < tol, rel
+
+
+def test_reuse_topology_keeps_tree_while_codes_are_unchanged(require_gpu):
+    """reuse_topology / rebuild_every: the tree and lists are kept while the
+    sorted Morton codes are unchanged and the budget allows; the reused state
+    evaluates like the prepared one on the same topology; changed codes or an
+    exhausted budget rebuild."""
+    from jaccpot_amd import FastMultipoleMethod
+
+    print("[gpu] reuse topology", flush=True)
+    x, m = ics.plummer(20000, 19)
+    fmm = FastMultipoleMethod(precision="fp64", theta=0.6, softening=1e-3, reuse_topology=True, rebuild_every=3)
+    s0 = fmm.prepare_state(x, m, leaf_size=16, max_order=4)
+    a0 = fmm.evaluate_prepared_state(s0)
+    # a tiny displacement that leaves every Morton code unchanged
+    x1 = x + 1e-13
+    s1 = fmm.prepare_state(x1, m, leaf_size=16, max_order=4)
+    assert fmm._reuse["count"] == 1 and s1.far_src is s0.far_src and s1 is not s0
+    a1 = fmm.evaluate_prepared_state(s1)
+    fresh = FastMultipoleMethod(precision="fp64", theta=0.6, softening=1e-3)
+    f0 = fresh.prepare_state(x, m, leaf_size=16, max_order=4)
+    fresh.refresh_prepared_state(f0, torch.as_tensor(x1, device="cuda"), torch.as_tensor(m, device="cuda"))
+    ref1 = fresh.evaluate_prepared_state(f0)
+    assert np.array_equal(a1, ref1)
+    assert np.array_equal(fmm.evaluate_prepared_state(s0), a0)  # the earlier state is untouched
+    s2 = fmm.prepare_state(x1, m, leaf_size=16, max_order=4)
+    assert fmm._reuse["count"] == 2 and s2.far_src is s0.far_src
+    s3 = fmm.prepare_state(x1, m, leaf_size=16, max_order=4)  # budget of rebuild_every - 1 = 2 reuses spent
+    assert fmm._reuse["count"] == 0 and s3.far_src is not s0.far_src
+    x4 = x1.copy()
+    x4[:100] *= 1.5  # moves particles to other Morton cells
+    s4 = fmm.prepare_state(x4, m, leaf_size=16, max_order=4)
+    assert fmm._reuse["count"] == 0 and s4.far_src is not s3.far_src
+
+
+def test_reuse_prepared_state_by_identity(require_gpu):
+    from jaccpot_amd import FastMultipoleMethod
+
+    x, m = ics.uniform(3000, 4)
+    fmm = FastMultipoleMethod(precision="fp64", theta=0.6, softening=1e-3)
+    a = fmm.compute_accelerations(x, m, leaf_size=16, max_order=4)
+    st = fmm._last_state
+    b = fmm.compute_accelerations(x, m, leaf_size=16, max_order=4, reuse_prepared_state=True)
+    assert fmm._last_state is st and np.array_equal(a, b)
+    c = fmm.compute_accelerations(x.copy(), m, leaf_size=16, max_order=4, reuse_prepared_state=True)
+    assert fmm._last_state is not st and np.array_equal(a, c)
