@@ -253,37 +253,10 @@ class FastMultipoleMethod:
             topo = build_topology(x_host, m_host, leaf_size=int(leaf_size), theta=th, bounds=bounds,
                                   num_threads=self.num_threads)
         on_dev = topo.extra.get("device", {})
-        code = nat.dtype_code(dtype)
-        cs = nat.coeff_stride(p, dtype)
-
-        def i32(name):
-            d = on_dev.get(name)
-            if d is not None and d.numel() > 0:
-                return d
-            a = np.asarray(getattr(topo, name))
-            return torch.as_tensor(np.ascontiguousarray(a if a.size else np.zeros(1), np.int32)).to(dev)
-
-        nodes = topo.num_nodes
-        state = FMMPreparedState(
-            topology=topo, order=p, dtype=dtype, G=self.G, softening=self.softening, theta=th, device=dev,
-            input_kind=kind, input_dtype=in_dtype, input_device=in_dev, cs=cs,
-            perm=i32("perm"),
-            pos4=torch.empty((topo.num_particles, 4), dtype=dtype, device=dev),
-            node_range=i32("node_range"), left=i32("left"), right=i32("right"), parent=i32("parent"),
-            m2m_nodes=i32("m2m_nodes"), l2l_nodes=i32("l2l_nodes"), m2l_targets=i32("m2l_targets"),
-            far_off=i32("far_off"), far_src=i32("far_src"), near_off=i32("near_off"), near_src=i32("near_src"),
-            centers4=torch.zeros((nodes, 4), dtype=dtype, device=dev),
-            multipoles=torch.zeros((nodes, cs), dtype=dtype, device=dev),
-            locals=torch.zeros((nodes, cs), dtype=dtype, device=dev),
-            compensated=self.compensated,
-        )
-        if self.mixed_precision and dtype == torch.float32:
-            state.extra["mixed"] = self._mixed_schedule(topo, p, dev)
-        if prefers_flat(topo.num_far_pairs, int(topo.m2l_targets.size)):
-            state.extra["flat"] = build_flat_list(pair_targets(state), state.far_src[: topo.num_far_pairs], p, dtype,
-                                                  dev)
-        if self.split_near:
-            state.extra["p2p_items"] = build_p2p_items(topo.near_off, topo.leaf_size, dtype, dev)
+        state = self._state_from_topology(topo, p, dtype, th, kind, in_dtype, in_dev,
+                                          mixed=self.mixed_precision and dtype == torch.float32,
+                                          split_near=self.split_near, G=self.G, softening=self.softening,
+                                          compensated=self.compensated)
         if self.adaptive_order:
             self._prepare_gears(state, xd, md)
         self.refresh_prepared_state(state, xd, md)
@@ -296,6 +269,58 @@ class FastMultipoleMethod:
                 codes, _ = sorted_morton(xd, bounds)
             self._reuse = {"key": key, "codes": codes, "state": state, "count": 0}
         return state
+
+    def _state_from_topology(self, topo: Topology, p: int, dtype: torch.dtype, th: float, kind: str, in_dtype,
+                             in_dev, *, mixed: bool, split_near: bool, G: float, softening: float,
+                             compensated: bool) -> FMMPreparedState:
+        """Device buffers and schedules of a prepared state for ``topo`` (expansions zero)."""
+        dev = self.device
+        on_dev = topo.extra.get("device", {})
+        cs = nat.coeff_stride(p, dtype)
+
+        def i32(name):
+            d = on_dev.get(name)
+            if d is not None and d.numel() > 0:
+                return d
+            a = np.asarray(getattr(topo, name))
+            return torch.as_tensor(np.ascontiguousarray(a if a.size else np.zeros(1), np.int32)).to(dev)
+
+        nodes = topo.num_nodes
+        state = FMMPreparedState(
+            topology=topo, order=p, dtype=dtype, G=G, softening=softening, theta=th, device=dev,
+            input_kind=kind, input_dtype=in_dtype, input_device=in_dev, cs=cs,
+            perm=i32("perm"),
+            pos4=torch.empty((topo.num_particles, 4), dtype=dtype, device=dev),
+            node_range=i32("node_range"), left=i32("left"), right=i32("right"), parent=i32("parent"),
+            m2m_nodes=i32("m2m_nodes"), l2l_nodes=i32("l2l_nodes"), m2l_targets=i32("m2l_targets"),
+            far_off=i32("far_off"), far_src=i32("far_src"), near_off=i32("near_off"), near_src=i32("near_src"),
+            centers4=torch.zeros((nodes, 4), dtype=dtype, device=dev),
+            multipoles=torch.zeros((nodes, cs), dtype=dtype, device=dev),
+            locals=torch.zeros((nodes, cs), dtype=dtype, device=dev),
+            compensated=compensated,
+        )
+        if mixed:
+            state.extra["mixed"] = self._mixed_schedule(topo, p, dev)
+        if prefers_flat(topo.num_far_pairs, int(topo.m2l_targets.size)):
+            state.extra["flat"] = build_flat_list(pair_targets(state), state.far_src[: topo.num_far_pairs], p, dtype,
+                                                  dev)
+        if split_near:
+            state.extra["p2p_items"] = build_p2p_items(topo.near_off, topo.leaf_size, dtype, dev)
+        return state
+
+    def save_prepared_state(self, state: FMMPreparedState, path) -> None:
+        """Write a prepared state to ``path`` (.npz: topology, tree-order particles,
+        centres, expansions, per-pair gears; no pickled objects)."""
+        from .persist import save_prepared_state
+
+        save_prepared_state(state, path)
+
+    def load_prepared_state(self, path) -> FMMPreparedState:
+        """Read a state written by ``save_prepared_state`` onto this solver's device;
+        it evaluates exactly as the saved one and can be refreshed like it."""
+        from .persist import load_prepared_state
+
+        return load_prepared_state(self, path)
 
     def _try_reuse(self, key, xd, md, bounds, kind, in_dtype, in_dev) -> Optional[FMMPreparedState]:
         """A state on the cached tree and lists when the new sorted Morton codes

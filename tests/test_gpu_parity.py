@@ -426,3 +426,35 @@ def test_reuse_prepared_state_by_identity(require_gpu):
     assert fmm._last_state is st and np.array_equal(a, b)
     c = fmm.compute_accelerations(x.copy(), m, leaf_size=16, max_order=4, reuse_prepared_state=True)
     assert fmm._last_state is not st and np.array_equal(a, c)
+
+
+@pytest.mark.parametrize("learned", [False, True], ids=["plain", "learned_mixed"])
+def test_saved_state_evaluates_identically(learned, tmp_path, require_gpu):
+    """save_prepared_state / load_prepared_state: the loaded state evaluates bit
+    for bit like the saved one, and after a refresh with new positions too."""
+    from jaccpot_amd import FastMultipoleMethod
+
+    print(f"[gpu] save/load prepared state learned={learned}", flush=True)
+    x, m = ics.plummer(20000, 21)
+    x, m = x.astype(np.float32), m.astype(np.float32)
+    kw, leaf, p = {}, 16, 4
+    if learned:
+        from jaccpot_amd.train_order import load, weights_path
+
+        kw = dict(mixed_precision=True, adaptive_order=True, p_gears=(3, 6, 8),
+                  order_model=load(weights_path((3, 6, 8))), gear_tolerance=-1.5)
+        leaf, p = 8, 8
+    fmm = FastMultipoleMethod(precision="fp32", theta=0.8, softening=1e-3, **kw)
+    st = fmm.prepare_state(x, m, leaf_size=leaf, max_order=p)
+    a, pa = fmm.evaluate_prepared_state(st, return_potential=True)
+    path = tmp_path / "state.npz"
+    fmm.save_prepared_state(st, path)
+    other = FastMultipoleMethod(precision="fp32", softening=0.5)  # settings come from the file
+    st2 = other.load_prepared_state(path)
+    b, pb = other.evaluate_prepared_state(st2, return_potential=True)
+    assert np.array_equal(a, b) and np.array_equal(pa, pb)
+    x2 = torch.as_tensor(x * np.float32(1.001), device="cuda")
+    md = torch.as_tensor(m, device="cuda")
+    fmm.refresh_prepared_state(st, x2, md)
+    other.refresh_prepared_state(st2, x2, md)
+    assert np.array_equal(fmm.evaluate_prepared_state(st), other.evaluate_prepared_state(st2))

@@ -42,3 +42,31 @@ def test_no_items_when_every_leaf_fits():
 
     near_off = np.arange(0, 101, 10)
     assert build_p2p_items(near_off, 8, torch.float32, "cpu", chunk_slots=2048) is None
+
+
+def test_prepared_state_file_holds_plain_arrays(tmp_path):
+    """The on-disk state is a JSON header plus plain arrays (no pickled objects)."""
+    import json
+    from types import SimpleNamespace
+
+    from jaccpot_amd.persist import FORMAT, TOPO_ARRAYS, save_prepared_state
+    from jaccpot_amd.topology import build_topology
+    from oracle import ics
+
+    x, m = ics.plummer(2000, 3)
+    topo = build_topology(x, m, leaf_size=16, theta=0.6)
+    nodes = topo.num_nodes
+    state = SimpleNamespace(topology=topo, order=4, dtype=torch.float64, G=1.0, softening=1e-3, theta=0.6,
+                            compensated=True, input_kind="numpy", input_dtype=np.dtype("float64"), extra={},
+                            pos4=torch.zeros((2000, 4), dtype=torch.float64),
+                            centers4=torch.zeros((nodes, 4), dtype=torch.float64),
+                            multipoles=torch.zeros((nodes, 28), dtype=torch.float64),
+                            locals=torch.ones((nodes, 28), dtype=torch.float64))
+    path = tmp_path / "state.npz"
+    save_prepared_state(state, path)
+    with np.load(path, allow_pickle=False) as z:
+        meta = json.loads(str(z["meta"]))
+        assert meta["format"] == FORMAT and meta["order"] == 4 and meta["gears"] is None
+        for k in TOPO_ARRAYS:
+            assert np.array_equal(z[f"topology.{k}"], np.asarray(getattr(topo, k)))
+        assert np.array_equal(z["locals"], np.ones((nodes, 28)))
