@@ -148,30 +148,14 @@ def main() -> int:
         flags = nat.EVAL_COMPENSATED
 
         def run_step(ev=None):
-            s = nat.current_stream()
-            if ev is not None:
-                ev[0].record()
-            nat.call("jfm_gather_sorted", s, code, code, xd.data_ptr(), md.data_ptr(), state.perm.data_ptr(), N,
-                     state.pos4.data_ptr())
-            fmm._upward(state, s, code)
-            if ev is not None:
-                ev[1].record()
-            fmm._downward(state, s, code)
-            if ev is not None:
-                ev[2].record()
-            nat.call("jfm_l2p", s, code, order, state.pos4.data_ptr(), N, leaf_size, topo.num_internal,
-                     state.locals.data_ptr(), state.centers4.data_ptr(), state.G, state.perm.data_ptr(),
-                     acc.data_ptr(), None)
-            if ev is not None:
-                ev[3].record()
-            fmm._p2p_device(state, acc, None, flags, accumulate=True)
-            if ev is not None:
-                ev[4].record()
+            # gather, then P2P on a second stream while upward + downward run, then L2P adds the far field
+            fmm.step_device(state, xd, md, acc, None, events=ev)
 
         for _ in range(args.warmup):
             run_step()
         torch.cuda.synchronize()
-        events = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(args.steps)]
+        events = [{k: [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+                   for k in ("upward", "downward", "l2p", "p2p")} for _ in range(args.steps)]
         torch.cuda.synchronize()
         t_start = time.perf_counter()
         for k in range(args.steps):
@@ -181,10 +165,8 @@ def main() -> int:
         ms_per_step = 1e3 * elapsed / args.steps
         stage = {"upward": [], "downward": [], "l2p": [], "p2p": []}
         for ev in events:
-            stage["upward"].append(ev[0].elapsed_time(ev[1]))
-            stage["downward"].append(ev[1].elapsed_time(ev[2]))
-            stage["l2p"].append(ev[2].elapsed_time(ev[3]))
-            stage["p2p"].append(ev[3].elapsed_time(ev[4]))
+            for k in stage:
+                stage[k].append(ev[k][0].elapsed_time(ev[k][1]))
         stage_ms = {k: float(np.mean(v)) for k, v in stage.items()}
         log(f"[{tag}] eager {ms_per_step:.3f} ms/step; stages (ms) "
             f"{json.dumps({k: round(v, 3) for k, v in stage_ms.items()})}")

@@ -272,13 +272,14 @@ int32_t jfm_l2l_top(void* stream, int32_t p, const int32_t* nodes, const int32_t
                     const int32_t* lev_off_host, const int32_t* top_nodes, int32_t ntop, const int32_t* parent,
                     const int32_t* shadow, const void* centers4, const void* centers64, void* locals, void* loc64);
 
-/* L2P far field only: writes rows out_index[i] of acc [rows][3] (and pot) with
- * a = -G grad_delta phi, pot = -G phi, delta = centre - x, for every tree slot i.
+/* L2P far field only: writes (accumulate == 0) or adds to (accumulate != 0)
+ * rows out_index[i] of acc [rows][3] (and pot) with a = -G grad_delta phi,
+ * pot = -G phi, delta = centre - x, for every tree slot i.
  * Replaces _evaluate_local_expansions_for_particles (runtime/kernels/_evaluate.py:2106-2485)
  * and evaluate_local_real_with_grad (operators/real_p2m_l2p.py:494-547). */
 int32_t jfm_l2p(void* stream, int32_t dtype, int32_t p, const void* pos4, int32_t n, int32_t leaf_size,
                 int32_t num_internal, const void* locals, const void* centers4, double G, const int32_t* out_index,
-                void* acc, void* pot);
+                void* acc, void* pot, int32_t accumulate);
 
 /* P2P near field only (self leaf i != j plus near leaves), leaf_size <= 256.
  * flags: JFM_EVAL_POTENTIAL, JFM_EVAL_COMPENSATED.  accumulate != 0 adds into

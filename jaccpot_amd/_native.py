@@ -25,7 +25,7 @@ EVAL_FAR = 2
 EVAL_POTENTIAL = 4
 EVAL_COMPENSATED = 8
 MAX_ORDER = 8
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 _i32p = POINTER(c_int32)
 _f64p = POINTER(c_double)
@@ -83,7 +83,7 @@ _SIGNATURES = {
     "jfm_l2l_top": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_void_p,
                               c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "jfm_l2p": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_int32, c_int32, c_int32, c_void_p, c_void_p,
-                          c_double, c_void_p, c_void_p, c_void_p]),
+                          c_double, c_void_p, c_void_p, c_void_p, c_int32]),
     "jfm_p2p": (c_int32, [c_void_p, c_int32, c_void_p, c_int32, c_int32, c_int32, c_void_p, c_void_p, c_double,
                           c_double, c_void_p, c_void_p, c_void_p, c_int32, c_int32]),
     "jfm_p2p_items": (c_int32, [c_void_p, c_int32, c_void_p, c_int32, c_int32, c_int32, c_int32, c_void_p, c_void_p,

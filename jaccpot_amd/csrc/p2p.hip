@@ -365,7 +365,7 @@ extern "C" {
 
 int32_t jfm_l2p(void* stream, int32_t dtype, int32_t p, const void* pos4, int32_t n, int32_t leaf_size,
                 int32_t num_internal, const void* locals, const void* centers4, double G, const int32_t* out_index,
-                void* acc, void* pot);
+                void* acc, void* pot, int32_t accumulate);
 
 int32_t jfm_p2p(void* stream, int32_t dtype, const void* pos4, int32_t n, int32_t num_leaves, int32_t leaf_size,
                 const int32_t* near_off, const int32_t* near_src, double eps2, double G, const int32_t* out_index,
@@ -455,7 +455,7 @@ int32_t jfm_evaluate(void* stream, int32_t dtype, int32_t p, const void* pos4, i
   void* potp = (flags & JFM_EVAL_POTENTIAL) ? pot : nullptr;
   if (far) {
     int32_t rc = jfm_l2p(stream, dtype, p, pos4, n, leaf_size, num_internal, locals, centers4, G, out_index, acc,
-                         potp);
+                         potp, 0);
     if (rc != JFM_OK) return rc;
   }
   if (near)

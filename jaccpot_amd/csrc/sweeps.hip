@@ -255,14 +255,14 @@ __global__ __launch_bounds__(kWave) void l2l_kernel(const int32_t* __restrict__ 
 }
 
 // One thread per particle: far field from its leaf's local expansion.
-// a = -G grad_delta phi, pot = -G phi, delta = centre - x.  Writes (not adds)
-// the output row.
+// a = -G grad_delta phi, pot = -G phi, delta = centre - x.  Writes the output
+// row, or adds to it (accumulate != 0, after the near field).
 template <typename T, int P>
 __global__ __launch_bounds__(256) void l2p_kernel(const T* __restrict__ pos4, int32_t n, int32_t leaf_size,
                                                   int32_t num_internal, const T* __restrict__ locals, int32_t cs,
                                                   const T* __restrict__ centers4, T G,
                                                   const int32_t* __restrict__ out_index, T* __restrict__ acc,
-                                                  T* __restrict__ pot) {
+                                                  T* __restrict__ pot, int32_t accumulate) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int node = num_internal + i / leaf_size;
@@ -272,10 +272,17 @@ __global__ __launch_bounds__(256) void l2p_kernel(const T* __restrict__ pos4, in
   T phi, gx, gy, gz;
   l2p_eval<T, P>([F](int j) { return F[j]; }, c.x - q.x, c.y - q.y, c.z - q.z, phi, gx, gy, gz);
   const int64_t row = out_index ? out_index[i] : i;
-  acc[3 * row + 0] = -G * gx;
-  acc[3 * row + 1] = -G * gy;
-  acc[3 * row + 2] = -G * gz;
-  if (pot) pot[row] = -G * phi;
+  if (accumulate) {
+    acc[3 * row + 0] += -G * gx;
+    acc[3 * row + 1] += -G * gy;
+    acc[3 * row + 2] += -G * gz;
+    if (pot) pot[row] += -G * phi;
+  } else {
+    acc[3 * row + 0] = -G * gx;
+    acc[3 * row + 1] = -G * gy;
+    acc[3 * row + 2] = -G * gz;
+    if (pot) pot[row] = -G * phi;
+  }
 }
 
 // Mixed precision: the internal nodes of the top three levels (depth <= 2,
@@ -552,7 +559,7 @@ int32_t jfm_l2l_top(void* stream, int32_t p, const int32_t* nodes, const int32_t
 
 int32_t jfm_l2p(void* stream, int32_t dtype, int32_t p, const void* pos4, int32_t n, int32_t leaf_size,
                 int32_t num_internal, const void* locals, const void* centers4, double G, const int32_t* out_index,
-                void* acc, void* pot) {
+                void* acc, void* pot, int32_t accumulate) {
   if (n < 1 || leaf_size < 1) return fail(JFM_ERR_ARG, "jfm_l2p: bad sizes");
   if (!pos4 || !locals || !centers4 || !acc) return fail(JFM_ERR_ARG, "jfm_l2p: NULL buffer");
   hipStream_t s = as_stream(stream);
@@ -560,7 +567,8 @@ int32_t jfm_l2p(void* stream, int32_t dtype, int32_t p, const void* pos4, int32_
     const int cs = coef_stride(P, sizeof(T));
     l2p_kernel<T, P><<<blocks_for(n, 256), 256, 0, s>>>((const T*)pos4, n, leaf_size, num_internal,
                                                         (const T*)locals, cs, (const T*)centers4,
-                                                        static_cast<T>(G), out_index, (T*)acc, (T*)pot);
+                                                        static_cast<T>(G), out_index, (T*)acc, (T*)pot,
+                                                        accumulate);
   }));
   return check_launch("jfm_l2p");
 }

@@ -504,7 +504,7 @@ class FastMultipoleMethod:
         if far:
             nat.call("jfm_l2p", s, code, state.order, state.pos4.data_ptr(), topo.num_particles, topo.leaf_size,
                      topo.num_internal, state.locals.data_ptr(), state.centers4.data_ptr(), state.G,
-                     state.perm.data_ptr(), acc.data_ptr(), potp)
+                     state.perm.data_ptr(), acc.data_ptr(), potp, 0)
         if flags & nat.EVAL_NEAR:
             self._p2p_items(state, items, acc, pot, flags, accumulate=far)
 
@@ -525,6 +525,59 @@ class FastMultipoleMethod:
                  items.num_split, items.split_leaf.data_ptr(), items.split_p0.data_ptr(), items.split_p1.data_ptr(),
                  items.partials.data_ptr(), state.softening ** 2, state.G, state.perm.data_ptr(), acc.data_ptr(),
                  potp, flags, 1 if accumulate else 0)
+
+    def step_device(self, state: FMMPreparedState, positions: torch.Tensor, masses: torch.Tensor,
+                    acc: torch.Tensor, pot: Optional[torch.Tensor] = None, events: Optional[dict] = None) -> None:
+        """One fixed-topology step (refresh_prepared_state + the evaluation) with the
+        near field overlapped: after the gather, P2P runs on a second stream
+        while the upward and downward passes run on the current one, then L2P
+        adds the far field to the near-field rows (the two terms are the same
+        as in the sequential order, and one addition of two values does not
+        depend on their order, so the result is identical).  ``acc``/``pot``
+        are written in input order.  ``events`` (optional): dict of
+        torch.cuda.Event pairs 'upward', 'downward', 'l2p' (current stream)
+        and 'p2p' (its stream) recorded around each part."""
+        topo = state.topology
+        main = torch.cuda.current_stream(state.device)
+        s = nat.current_stream()
+        code = nat.dtype_code(state.dtype)
+        src_code = nat.dtype_code(positions.dtype)
+        positions = positions.contiguous()
+        masses = masses.to(positions.dtype).contiguous()
+
+        def mark(name, k):
+            if events is not None:
+                events[name][k].record()
+
+        mark("upward", 0)
+        nat.call("jfm_gather_sorted", s, src_code, code, positions.data_ptr(), masses.data_ptr(),
+                 state.perm.data_ptr(), topo.num_particles, state.pos4.data_ptr())
+        side = self._side_stream(state.device)
+        side.wait_stream(main)
+        flags = nat.EVAL_COMPENSATED if state.compensated else 0
+        with torch.cuda.stream(side):
+            mark("p2p", 0)
+            self._p2p_device(state, acc, pot, flags, accumulate=False)
+            mark("p2p", 1)
+        self._upward(state, s, code)
+        mark("upward", 1)
+        mark("downward", 0)
+        self._downward(state, s, code)
+        mark("downward", 1)
+        main.wait_stream(side)
+        mark("l2p", 0)
+        nat.call("jfm_l2p", s, code, state.order, state.pos4.data_ptr(), topo.num_particles, topo.leaf_size,
+                 topo.num_internal, state.locals.data_ptr(), state.centers4.data_ptr(), state.G,
+                 state.perm.data_ptr(), acc.data_ptr(), pot.data_ptr() if pot is not None else None, 1)
+        mark("l2p", 1)
+
+    def _side_stream(self, device) -> torch.cuda.Stream:
+        st = getattr(self, "_streams", None)
+        if st is None:
+            st = self._streams = {}
+        if device not in st:
+            st[device] = torch.cuda.Stream(device=device)  # default priority
+        return st[device]
 
     def _p2p_device(self, state: FMMPreparedState, acc: torch.Tensor, pot: Optional[torch.Tensor], flags: int,
                     accumulate: bool) -> None:
@@ -630,8 +683,7 @@ class EvaluationGraph:
         self.pot = torch.empty((n,), dtype=state.dtype, device=state.device) if return_potential else None
 
         def step():
-            fmm.refresh_prepared_state(state, self.positions, self.masses)
-            fmm._evaluate_device(state, self.acc, self.pot)
+            fmm.step_device(state, self.positions, self.masses, self.acc, self.pot)
 
         side = torch.cuda.Stream(device=state.device)
         side.wait_stream(torch.cuda.current_stream(state.device))

@@ -458,3 +458,35 @@ def test_saved_state_evaluates_identically(learned, tmp_path, require_gpu):
     fmm.refresh_prepared_state(st, x2, md)
     other.refresh_prepared_state(st2, x2, md)
     assert np.array_equal(fmm.evaluate_prepared_state(st), other.evaluate_prepared_state(st2))
+
+
+@pytest.mark.parametrize("learned", [False, True], ids=["plain", "learned_mixed"])
+def test_overlapped_step_matches_sequential(learned, require_gpu):
+    """step_device (P2P on a second stream during the upward/downward passes,
+    L2P added afterwards) equals refresh_prepared_state + evaluation bit for bit."""
+    from jaccpot_amd import FastMultipoleMethod
+
+    print(f"[gpu] overlapped step learned={learned}", flush=True)
+    x, m = ics.plummer(50000, 23)
+    x, m = x.astype(np.float32), m.astype(np.float32)
+    kw, leaf, p = {}, 16, 4
+    if learned:
+        from jaccpot_amd.train_order import load, weights_path
+
+        kw = dict(mixed_precision=True, adaptive_order=True, p_gears=(3, 6, 8),
+                  order_model=load(weights_path((3, 6, 8))), gear_tolerance=-1.5)
+        leaf, p = 8, 8
+    fmm = FastMultipoleMethod(precision="fp32", theta=0.8, softening=1e-3, **kw)
+    st = fmm.prepare_state(x, m, leaf_size=leaf, max_order=p)
+    xd = torch.as_tensor(x * np.float32(1.0005), device="cuda")
+    md = torch.as_tensor(m, device="cuda")
+    n = x.shape[0]
+    acc_o = torch.empty((n, 3), dtype=torch.float32, device="cuda")
+    pot_o = torch.empty((n,), dtype=torch.float32, device="cuda")
+    fmm.step_device(st, xd, md, acc_o, pot_o)
+    acc_s = torch.empty_like(acc_o)
+    pot_s = torch.empty_like(pot_o)
+    fmm.refresh_prepared_state(st, xd, md)
+    fmm._evaluate_device(st, acc_s, pot_s)
+    torch.cuda.synchronize()
+    assert torch.equal(acc_o, acc_s) and torch.equal(pot_o, pot_s)
