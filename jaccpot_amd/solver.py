@@ -527,7 +527,8 @@ class FastMultipoleMethod:
                  potp, flags, 1 if accumulate else 0)
 
     def step_device(self, state: FMMPreparedState, positions: torch.Tensor, masses: torch.Tensor,
-                    acc: torch.Tensor, pot: Optional[torch.Tensor] = None, events: Optional[dict] = None) -> None:
+                    acc: torch.Tensor, pot: Optional[torch.Tensor] = None, events: Optional[dict] = None,
+                    overlap: bool = True) -> None:
         """One fixed-topology step (refresh_prepared_state + the evaluation) with the
         near field overlapped: after the gather, P2P runs on a second stream
         while the upward and downward passes run on the current one, then L2P
@@ -536,7 +537,8 @@ class FastMultipoleMethod:
         depend on their order, so the result is identical).  ``acc``/``pot``
         are written in input order.  ``events`` (optional): dict of
         torch.cuda.Event pairs 'upward', 'downward', 'l2p' (current stream)
-        and 'p2p' (its stream) recorded around each part."""
+        and 'p2p' (its stream) recorded around each part.  ``overlap=False`` runs
+        the same launches in the same order on the current stream only."""
         topo = state.topology
         main = torch.cuda.current_stream(state.device)
         s = nat.current_stream()
@@ -552,7 +554,7 @@ class FastMultipoleMethod:
         mark("upward", 0)
         nat.call("jfm_gather_sorted", s, src_code, code, positions.data_ptr(), masses.data_ptr(),
                  state.perm.data_ptr(), topo.num_particles, state.pos4.data_ptr())
-        side = self._side_stream(state.device)
+        side = self._side_stream(state.device) if overlap else main
         side.wait_stream(main)
         flags = nat.EVAL_COMPENSATED if state.compensated else 0
         with torch.cuda.stream(side):
@@ -564,7 +566,8 @@ class FastMultipoleMethod:
         mark("downward", 0)
         self._downward(state, s, code)
         mark("downward", 1)
-        main.wait_stream(side)
+        if overlap:
+            main.wait_stream(side)
         mark("l2p", 0)
         nat.call("jfm_l2p", s, code, state.order, state.pos4.data_ptr(), topo.num_particles, topo.leaf_size,
                  topo.num_internal, state.locals.data_ptr(), state.centers4.data_ptr(), state.G,
