@@ -538,7 +538,8 @@ class FastMultipoleMethod:
         are written in input order.  ``events`` (optional): dict of
         torch.cuda.Event pairs 'upward', 'downward', 'l2p' (current stream)
         and 'p2p' (its stream) recorded around each part.  ``overlap=False`` runs
-        the same launches in the same order on the current stream only."""
+        the same launches on the current stream only, P2P after the downward
+        pass."""
         topo = state.topology
         main = torch.cuda.current_stream(state.device)
         s = nat.current_stream()
@@ -554,13 +555,18 @@ class FastMultipoleMethod:
         mark("upward", 0)
         nat.call("jfm_gather_sorted", s, src_code, code, positions.data_ptr(), masses.data_ptr(),
                  state.perm.data_ptr(), topo.num_particles, state.pos4.data_ptr())
-        side = self._side_stream(state.device) if overlap else main
-        side.wait_stream(main)
         flags = nat.EVAL_COMPENSATED if state.compensated else 0
-        with torch.cuda.stream(side):
+
+        def near():
             mark("p2p", 0)
             self._p2p_device(state, acc, pot, flags, accumulate=False)
             mark("p2p", 1)
+
+        if overlap:
+            side = self._side_stream(state.device)
+            side.wait_stream(main)
+            with torch.cuda.stream(side):
+                near()
         self._upward(state, s, code)
         mark("upward", 1)
         mark("downward", 0)
@@ -568,6 +574,8 @@ class FastMultipoleMethod:
         mark("downward", 1)
         if overlap:
             main.wait_stream(side)
+        else:
+            near()
         mark("l2p", 0)
         nat.call("jfm_l2p", s, code, state.order, state.pos4.data_ptr(), topo.num_particles, topo.leaf_size,
                  topo.num_internal, state.locals.data_ptr(), state.centers4.data_ptr(), state.G,
