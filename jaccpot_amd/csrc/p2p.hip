@@ -62,9 +62,10 @@ __device__ __forceinline__ void kahan_add(T& s, T& c, T v) {
 // loads one tile ahead of position loads, so neither is waited for before the
 // interactions of the current tile.  For leaf_size == WT (power-of-two
 // leaves) the slot decode is a shift and a mask.
-template <int WT, int LOADS, int STRIDE>
+// FIXED: leaf_size == WT, known at compile time (shift and mask, no branch).
+template <int WT, bool FIXED>
 __device__ __forceinline__ void slot_decode(int leaf_size, int slot, int& qi, int& o) {
-  if (leaf_size == WT) {
+  if constexpr (FIXED) {
     qi = static_cast<int>(static_cast<unsigned>(slot) / WT);
     o = static_cast<int>(static_cast<unsigned>(slot) % WT);
   } else {
@@ -73,7 +74,7 @@ __device__ __forceinline__ void slot_decode(int leaf_size, int slot, int& qi, in
   }
 }
 
-template <int WT, int LOADS, int STRIDE>
+template <int WT, bool FIXED, int LOADS, int STRIDE>
 __device__ __forceinline__ void fetch_list(int leaf_size, const int32_t* __restrict__ near_list, int nnear,
                                            const int32_t* __restrict__ fallback, int first, int (&v)[LOADS]) {
   const int32_t* list = nnear > 0 ? near_list : fallback;
@@ -81,19 +82,19 @@ __device__ __forceinline__ void fetch_list(int leaf_size, const int32_t* __restr
 #pragma unroll
   for (int l = 0; l < LOADS; ++l) {
     int qi, o;
-    slot_decode<WT, LOADS, STRIDE>(leaf_size, first + l * STRIDE, qi, o);
+    slot_decode<WT, FIXED>(leaf_size, first + l * STRIDE, qi, o);
     v[l] = list[min(max(qi - 1, 0), hi)];
   }
 }
 
-template <typename T, int WT, int LOADS, int STRIDE>
+template <typename T, int WT, bool FIXED, int LOADS, int STRIDE>
 __device__ __forceinline__ void fetch_positions(const T* __restrict__ pos4, int n, int leaf_size, int leaf, int total,
                                                 int first, const int (&v)[LOADS], V4<T> (&out)[LOADS],
                                                 bool (&valid)[LOADS]) {
 #pragma unroll
   for (int l = 0; l < LOADS; ++l) {
     int qi, o;
-    slot_decode<WT, LOADS, STRIDE>(leaf_size, first + l * STRIDE, qi, o);
+    slot_decode<WT, FIXED>(leaf_size, first + l * STRIDE, qi, o);
     const int sl = qi > 0 ? v[l] : leaf;
     const int g = first + l * STRIDE < total ? sl * leaf_size + o : n;
     valid[l] = g < n;
@@ -118,7 +119,7 @@ struct P2PItems {
 };
 
 // SAFE: guard r^2 == 0 (unsoftened self term / coincident points).
-template <typename T, int WT, bool POT, bool COMP, bool SAFE>
+template <typename T, int WT, bool POT, bool COMP, bool SAFE, bool FIXED>
 __global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos4, int32_t n, int32_t leaf_size,
                                                        const int32_t* __restrict__ near_off,
                                                        const int32_t* __restrict__ near_src, T eps2, T G,
@@ -155,9 +156,9 @@ __global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos
   bool ok[LOADS];
   int lst[LOADS];  // near-list entries of the tile after `next`
   const int tid = threadIdx.x;
-  fetch_list<WT, LOADS, kThreads>(leaf_size, near_src + nb, nsl - 1, near_off, start + tid, lst);
-  fetch_positions<T, WT, LOADS, kThreads>(pos4, n, leaf_size, leaf, total, start + tid, lst, next, ok);
-  fetch_list<WT, LOADS, kThreads>(leaf_size, near_src + nb, nsl - 1, near_off, start + TILE + tid, lst);
+  fetch_list<WT, FIXED, LOADS, kThreads>(leaf_size, near_src + nb, nsl - 1, near_off, start + tid, lst);
+  fetch_positions<T, WT, FIXED, LOADS, kThreads>(pos4, n, leaf_size, leaf, total, start + tid, lst, next, ok);
+  fetch_list<WT, FIXED, LOADS, kThreads>(leaf_size, near_src + nb, nsl - 1, near_off, start + TILE + tid, lst);
 
   T sx = 0, sy = 0, sz = 0, sp = 0;
   T kx = 0, ky = 0, kz = 0, kp = 0;
@@ -172,8 +173,8 @@ __global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos
     __syncthreads();
     // unconditional (past the end they yield invalid slots): a branch here
     // would make the compiler wait for the loads at its join
-    fetch_positions<T, WT, LOADS, kThreads>(pos4, n, leaf_size, leaf, total, base + TILE + tid, lst, next, ok);
-    fetch_list<WT, LOADS, kThreads>(leaf_size, near_src + nb, nsl - 1, near_off, base + 2 * TILE + tid, lst);
+    fetch_positions<T, WT, FIXED, LOADS, kThreads>(pos4, n, leaf_size, leaf, total, base + TILE + tid, lst, next, ok);
+    fetch_list<WT, FIXED, LOADS, kThreads>(leaf_size, near_src + nb, nsl - 1, near_off, base + 2 * TILE + tid, lst);
     // Slots past `total` hold zero-mass sources, so every partition runs the
     // same (wave-uniform, scalar-controlled) trip count.
     const int trips = (min(TILE, total - base) + NPART - 1) / NPART;
@@ -312,10 +313,15 @@ int32_t launch_p2p_wt(hipStream_t s, const T* pos4, int32_t n, int32_t leaf_size
                       T* acc, T* pot, bool comp, bool accumulate, P2PItems<T> items) {
   const bool want_pot = pot != nullptr;
   const bool safe = !(eps2 > T(0));
-#define JFM_P2P_LAUNCH(POT, COMP, SAFE)                                                                  \
-  p2p_kernel<T, WT, POT, COMP, SAFE><<<num_blocks, kThreads, 0, s>>>(pos4, n, leaf_size, near_off, near_src, \
-                                                                     eps2, G, out_index, acc, pot,           \
-                                                                     accumulate ? 1 : 0, items)
+  // the common case (potential off, softened, compensated, leaf_size == WT) gets the
+  // compile-time slot decode; every other combination the general one
+#define JFM_P2P_LAUNCH(POT, COMP, SAFE)                                                                      \
+  if (!POT && COMP && !SAFE && leaf_size == WT)                                                              \
+    p2p_kernel<T, WT, POT, COMP, SAFE, true><<<num_blocks, kThreads, 0, s>>>(                                 \
+        pos4, n, leaf_size, near_off, near_src, eps2, G, out_index, acc, pot, accumulate ? 1 : 0, items);    \
+  else                                                                                                       \
+    p2p_kernel<T, WT, POT, COMP, SAFE, false><<<num_blocks, kThreads, 0, s>>>(                                \
+        pos4, n, leaf_size, near_off, near_src, eps2, G, out_index, acc, pot, accumulate ? 1 : 0, items)
   if (want_pot) {
     if (comp) {
       if (safe) JFM_P2P_LAUNCH(true, true, true); else JFM_P2P_LAUNCH(true, true, false);
