@@ -207,31 +207,22 @@ __global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos
       if (POT) sp += tp;
     }
   }
+  // partitions reduced pairwise in LDS (fixed tree order, every thread takes part)
   __syncthreads();
-  tile[threadIdx.x] = V4<T>{sx, sy, sz, sp};
-  tile[kThreads + threadIdx.x] = V4<T>{kx, ky, kz, kp};
+  tile[threadIdx.x] = V4<T>{sx - kx, sy - ky, sz - kz, sp - kp};
   __syncthreads();
-  if (part == 0 && i < cnt) {
-    T ax = 0, ay = 0, az = 0, ap = 0, cx = 0, cy = 0, cz = 0, cp = 0;
-    // partial unroll: a full one hoists 2 * NPART row loads into registers,
-    // which for narrow leaves (NPART = 16) costs occupancy in the whole kernel
-#pragma unroll 2
-    for (int q = 0; q < NPART; ++q) {
-      const V4<T> v = tile[q * WT + i];
-      const V4<T> c = tile[kThreads + q * WT + i];
-      ax += v.x;
-      ay += v.y;
-      az += v.z;
-      ap += v.w;
-      cx += c.x;
-      cy += c.y;
-      cz += c.z;
-      cp += c.w;
+#pragma unroll
+  for (int w = NPART / 2; w >= 1; w >>= 1) {
+    if (part < w) {
+      const V4<T> a = tile[threadIdx.x];
+      const V4<T> b = tile[threadIdx.x + w * WT];
+      tile[threadIdx.x] = V4<T>{a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w};
     }
-    ax -= cx;
-    ay -= cy;
-    az -= cz;
-    ap -= cp;
+    __syncthreads();
+  }
+  if (part == 0 && i < cnt) {
+    const V4<T> r = tile[i];
+    T ax = r.x, ay = r.y, az = r.z, ap = r.w;
     if (part_out >= 0) {
       if (part_out >= items.num_part) return;
       T* dst = items.partials + (static_cast<int64_t>(part_out) * leaf_size + i) * 4;
