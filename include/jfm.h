@@ -142,16 +142,21 @@ int32_t jfm_tree_level_geometry(void* stream, int32_t count, const int32_t* leve
  * scratch holds n * 3 doubles (the positions in tree order). */
 int32_t jfm_tree_radius(void* stream, int32_t nodes, int64_t n, const double* pos, const int32_t* perm,
                         const int32_t* node_range, const double* center, double* scratch, double* radius);
-/* One breadth-first step of the dual-tree walk: expands items_in [n_in][2]
- * into items_out, appending far pairs (target << 32 | source node) and near
- * pairs (target << 32 | source leaf) after counts[1] / counts[2] entries;
- * counts[0] (zeroed by the caller) receives the new item count.  A count
- * above its capacity means that output was truncated: re-run the step with
- * larger buffers and counts reset. */
-int32_t jfm_tree_walk_step(void* stream, int32_t I, const int32_t* items_in, int64_t n_in, const int32_t* left,
-                           const int32_t* right, const double* center, const double* radius, double theta,
-                           int32_t* items_out, int64_t cap_items, uint64_t* far_keys, int64_t cap_far,
-                           uint64_t* near_keys, int64_t cap_near, uint64_t* counts);
+/* One breadth-first step of the dual-tree walk in three calls.  classify:
+ * codes [n_in] (what item t emits) and counts3 [3][n_in] (items, far keys,
+ * near keys of item t); jfm_scan_i32 on each row of counts3 gives offsets3;
+ * emit: item t writes its child items at items_out[offsets3[0][t]], its far
+ * pairs (target << 32 | source node) at far_keys[far_base + offsets3[1][t]]
+ * and its near pairs (target << 32 | source leaf) at near_keys[near_base +
+ * offsets3[2][t]] (the caller sizes every output from the scanned totals). */
+int32_t jfm_tree_walk_classify(void* stream, int32_t I, const int32_t* items_in, int64_t n_in, const double* center,
+                               const double* radius, double theta, uint8_t* codes, int32_t* counts3);
+int32_t jfm_tree_walk_emit(void* stream, int32_t I, const int32_t* items_in, int64_t n_in, const int32_t* left,
+                           const int32_t* right, const uint8_t* codes, const int32_t* offsets3, int64_t far_base,
+                           int64_t near_base, int32_t* items_out, uint64_t* far_keys, uint64_t* near_keys);
+/* Exclusive prefix sum of int32 values; scratch holds jfm_scan_scratch_bytes(n). */
+int32_t jfm_scan_scratch_bytes(int64_t n, int64_t* bytes);
+int32_t jfm_scan_i32(void* stream, int64_t n, const int32_t* in, int32_t* out, void* scratch, int64_t scratch_bytes);
 /* Sorted (row << 32 | col) keys -> CSR off [rows+1], col [K]. */
 int32_t jfm_keys_to_csr(void* stream, const uint64_t* sorted_keys, int64_t K, int32_t rows, int32_t* off,
                         int32_t* col);

@@ -25,7 +25,7 @@ EVAL_FAR = 2
 EVAL_POTENTIAL = 4
 EVAL_COMPENSATED = 8
 MAX_ORDER = 8
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 _i32p = POINTER(c_int32)
 _f64p = POINTER(c_double)
@@ -54,8 +54,12 @@ _SIGNATURES = {
     "jfm_tree_leaf_geometry": (c_int32, [c_void_p, c_int32, c_int32] + [c_void_p] * 6),
     "jfm_tree_level_geometry": (c_int32, [c_void_p, c_int32] + [c_void_p] * 9),
     "jfm_tree_radius": (c_int32, [c_void_p, c_int32, c_int64] + [c_void_p] * 6),
-    "jfm_tree_walk_step": (c_int32, [c_void_p, c_int32, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
-                                     c_double, c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_int64, c_void_p]),
+    "jfm_tree_walk_classify": (c_int32, [c_void_p, c_int32, c_void_p, c_int64, c_void_p, c_void_p, c_double,
+                                         c_void_p, c_void_p]),
+    "jfm_tree_walk_emit": (c_int32, [c_void_p, c_int32, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
+                                     c_int64, c_int64, c_void_p, c_void_p, c_void_p]),
+    "jfm_scan_scratch_bytes": (c_int32, [c_int64, POINTER(c_int64)]),
+    "jfm_scan_i32": (c_int32, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int64]),
     "jfm_keys_to_csr": (c_int32, [c_void_p, c_void_p, c_int64, c_int32, c_void_p, c_void_p]),
     "jfm_key_bits": (c_int32, [ctypes.c_uint64]),
     "jfm_pair_rows": (c_int32, [c_void_p, c_int32, c_void_p, c_int64, c_void_p]),

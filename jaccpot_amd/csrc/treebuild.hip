@@ -25,15 +25,16 @@
 // Floating-point expressions are evaluated without contraction so every sum,
 // product, quotient and square root rounds exactly as the host's.
 //
-// Buffers are the caller's; the sorts take a caller-provided scratch
-// (jfm_sort_scratch_bytes).  The breadth-first walk step reports how many
-// items / pairs it produced; when a count exceeds the capacity passed in, the
-// step wrote only what fitted and the caller re-runs it with larger buffers.
+// Buffers are the caller's; the sorts and scans take a caller-provided
+// scratch (jfm_sort_scratch_bytes, jfm_scan_scratch_bytes).  A walk step is
+// classify (what each item emits), an exclusive scan of the counts, and emit
+// (each item writes at its offsets), so the caller sizes every output exactly.
 
 #pragma clang fp contract(off)
 
 #include <hip/hip_runtime.h>
 #include <hipcub/device/device_radix_sort.hpp>
+#include <hipcub/device/device_scan.hpp>
 
 #include <cmath>
 #include <cstdint>
@@ -347,86 +348,82 @@ __global__ void __launch_bounds__(1024) radius_big_kernel(const double* __restri
 // dual-tree walk, one breadth-first step
 // ---------------------------------------------------------------------------
 
-struct WalkOut {
-  int2* items;
-  int64_t cap_items;
-  uint64_t* far;
-  int64_t cap_far;
-  uint64_t* near;
-  int64_t cap_near;
-  unsigned long long* count;  // [3]: items, far, near
-};
+// Every frontier item is classified first (what it emits), the per-item
+// counts are exclusive-scanned, and a second pass writes each item's outputs
+// at its scanned offsets: no atomics, and the output order is a function of
+// the input order alone.
+enum WalkCode : uint8_t { kNone = 0, kSelf = 1, kFar = 2, kNear = 3, kSplitA = 4, kSplitB = 5 };
 
-// Reserve `mine` slots of a shared counter for every active lane with one
-// atomic per wave; returns this lane's first slot.
-__device__ inline int64_t wave_reserve(unsigned long long* counter, uint32_t mine) {
-  const int lane = threadIdx.x % 64;
-  const uint64_t active = __ballot(1);
-  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-  // mine is one of 0, 2, 3: two ballots give every lane's exclusive prefix
-  const uint64_t b2 = __ballot(mine == 2), b3 = __ballot(mine == 3);
-  const uint32_t before = 2u * __popcll(b2 & lt) + 3u * __popcll(b3 & lt);
-  const uint32_t total = 2u * __popcll(b2) + 3u * __popcll(b3);
-  const int leader = __ffsll(static_cast<long long>(active)) - 1;
-  unsigned long long base = 0;
-  if (lane == leader && total) base = atomicAdd(counter, static_cast<unsigned long long>(total));
-  base = __shfl(base, leader);
-  return static_cast<int64_t>(base) + before;
-}
-
-__global__ void __launch_bounds__(256) walk_step_kernel(const int2* __restrict__ in, int64_t n_in, int32_t I,
-                                                        const int32_t* __restrict__ left,
-                                                        const int32_t* __restrict__ right,
-                                                        const double* __restrict__ C, const double* __restrict__ R,
-                                                        double th2, WalkOut o) {
+__global__ void __launch_bounds__(256) walk_classify_kernel(const int2* __restrict__ in, int64_t n_in, int32_t I,
+                                                            const double* __restrict__ C,
+                                                            const double* __restrict__ R, double th2,
+                                                            uint8_t* __restrict__ code,
+                                                            int32_t* __restrict__ counts3) {
   const int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (t >= n_in) return;  // the lanes that stay all take part in the reservations below
+  if (t >= n_in) return;
   const int32_t a = in[t].x, b = in[t].y;
-  uint32_t n_items = 0, n_far = 0, n_near = 0;
-  int2 it0 = make_int2(0, 0), it1 = it0, it2 = it0;
+  uint8_t c;
   if (a == b) {
-    if (a < I) {
-      const int32_t la = left[a], ra = right[a];
-      it0 = make_int2(la, ra);
-      it1 = make_int2(ra, ra);
-      it2 = make_int2(la, la);
-      n_items = 3;
-    }
+    c = a < I ? kSelf : kNone;
   } else {
     const double dx = C[3 * a] - C[3 * b], dy = C[3 * a + 1] - C[3 * b + 1], dz = C[3 * a + 2] - C[3 * b + 2];
     const double d2 = dx * dx + dy * dy + dz * dz;
     const double s = R[a] + R[b];
     const bool al = a >= I, bl = b >= I;
-    if (d2 > 0.0 && s * s <= th2 * d2) {
-      n_far = 2;
-    } else if (al && bl) {
-      n_near = 2;
-    } else if (bl || (!al && R[a] >= R[b])) {
-      it0 = make_int2(right[a], b);
-      it1 = make_int2(left[a], b);
-      n_items = 2;
-    } else {
-      it0 = make_int2(a, right[b]);
-      it1 = make_int2(a, left[b]);
-      n_items = 2;
+    if (d2 > 0.0 && s * s <= th2 * d2) c = kFar;
+    else if (al && bl) c = kNear;
+    else if (bl || (!al && R[a] >= R[b])) c = kSplitA;
+    else c = kSplitB;
+  }
+  code[t] = c;
+  counts3[t] = c == kSelf ? 3 : (c == kSplitA || c == kSplitB) ? 2 : 0;
+  counts3[n_in + t] = c == kFar ? 2 : 0;
+  counts3[2 * n_in + t] = c == kNear ? 2 : 0;
+}
+
+__global__ void __launch_bounds__(256) walk_emit_kernel(const int2* __restrict__ in, int64_t n_in, int32_t I,
+                                                        const int32_t* __restrict__ left,
+                                                        const int32_t* __restrict__ right,
+                                                        const uint8_t* __restrict__ code,
+                                                        const int32_t* __restrict__ offsets3, int64_t far_base,
+                                                        int64_t near_base, int2* __restrict__ items_out,
+                                                        uint64_t* __restrict__ far, uint64_t* __restrict__ near) {
+  const int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (t >= n_in) return;
+  const int32_t a = in[t].x, b = in[t].y;
+  const uint8_t c = code[t];
+  const int64_t oi = offsets3[t];
+  switch (c) {
+    case kSelf: {
+      const int32_t la = left[a], ra = right[a];
+      items_out[oi] = make_int2(la, ra);
+      items_out[oi + 1] = make_int2(ra, ra);
+      items_out[oi + 2] = make_int2(la, la);
+      break;
     }
-  }
-  const int64_t at_i = wave_reserve(&o.count[0], n_items);
-  const int64_t at_f = wave_reserve(&o.count[1], n_far);
-  const int64_t at_n = wave_reserve(&o.count[2], n_near);
-  if (n_items && at_i + n_items <= o.cap_items) {
-    o.items[at_i] = it0;
-    o.items[at_i + 1] = it1;
-    if (n_items == 3) o.items[at_i + 2] = it2;
-  }
-  if (n_far && at_f + 2 <= o.cap_far) {
-    o.far[at_f] = (uint64_t(uint32_t(a)) << 32) | uint32_t(b);
-    o.far[at_f + 1] = (uint64_t(uint32_t(b)) << 32) | uint32_t(a);
-  }
-  if (n_near && at_n + 2 <= o.cap_near) {
-    const uint32_t ta = uint32_t(a - I), tb = uint32_t(b - I);
-    o.near[at_n] = (uint64_t(ta) << 32) | tb;
-    o.near[at_n + 1] = (uint64_t(tb) << 32) | ta;
+    case kFar: {
+      const int64_t o = far_base + offsets3[n_in + t];
+      far[o] = (uint64_t(uint32_t(a)) << 32) | uint32_t(b);
+      far[o + 1] = (uint64_t(uint32_t(b)) << 32) | uint32_t(a);
+      break;
+    }
+    case kNear: {
+      const int64_t o = near_base + offsets3[2 * n_in + t];
+      const uint32_t ta = uint32_t(a - I), tb = uint32_t(b - I);
+      near[o] = (uint64_t(ta) << 32) | tb;
+      near[o + 1] = (uint64_t(tb) << 32) | ta;
+      break;
+    }
+    case kSplitA:
+      items_out[oi] = make_int2(right[a], b);
+      items_out[oi + 1] = make_int2(left[a], b);
+      break;
+    case kSplitB:
+      items_out[oi] = make_int2(a, right[b]);
+      items_out[oi + 1] = make_int2(a, left[b]);
+      break;
+    default:
+      break;
   }
 }
 
@@ -593,19 +590,51 @@ int32_t jfm_tree_radius(void* stream, int32_t nodes, int64_t n, const double* po
   return jfm::check_launch("jfm_tree_radius");
 }
 
-int32_t jfm_tree_walk_step(void* stream, int32_t I, const int32_t* items_in, int64_t n_in, const int32_t* left,
-                           const int32_t* right, const double* center, const double* radius, double theta,
-                           int32_t* items_out, int64_t cap_items, uint64_t* far_keys, int64_t cap_far,
-                           uint64_t* near_keys, int64_t cap_near, uint64_t* counts) {
-  if (n_in < 0 || I < 0 || !center || !radius || !counts || (n_in > 0 && !items_in) || (I > 0 && (!left || !right)) ||
-      cap_items < 0 || cap_far < 0 || cap_near < 0 || !(theta > 0.0))
-    return fail(JFM_ERR_ARG, "jfm_tree_walk_step: bad argument");
+int32_t jfm_tree_walk_classify(void* stream, int32_t I, const int32_t* items_in, int64_t n_in, const double* center,
+                               const double* radius, double theta, uint8_t* codes, int32_t* counts3) {
+  if (n_in < 0 || n_in > INT32_MAX || I < 0 || !center || !radius || !(theta > 0.0) ||
+      (n_in > 0 && (!items_in || !codes || !counts3)))
+    return fail(JFM_ERR_ARG, "jfm_tree_walk_classify: bad argument");
   if (n_in == 0) return JFM_OK;
-  jfm::WalkOut o{reinterpret_cast<int2*>(items_out), cap_items, far_keys,
-                 cap_far, near_keys, cap_near, reinterpret_cast<unsigned long long*>(counts)};
-  hipLaunchKernelGGL(jfm::walk_step_kernel, dim3(jfm::grid_for(n_in)), dim3(256), 0, jfm::as_stream(stream),
-                     reinterpret_cast<const int2*>(items_in), n_in, I, left, right, center, radius, theta * theta, o);
-  return jfm::check_launch("jfm_tree_walk_step");
+  hipLaunchKernelGGL(jfm::walk_classify_kernel, dim3(jfm::grid_for(n_in)), dim3(256), 0, jfm::as_stream(stream),
+                     reinterpret_cast<const int2*>(items_in), n_in, I, center, radius, theta * theta, codes, counts3);
+  return jfm::check_launch("jfm_tree_walk_classify");
+}
+
+int32_t jfm_tree_walk_emit(void* stream, int32_t I, const int32_t* items_in, int64_t n_in, const int32_t* left,
+                           const int32_t* right, const uint8_t* codes, const int32_t* offsets3, int64_t far_base,
+                           int64_t near_base, int32_t* items_out, uint64_t* far_keys, uint64_t* near_keys) {
+  if (n_in < 0 || n_in > INT32_MAX || I < 0 || far_base < 0 || near_base < 0 ||
+      (n_in > 0 && (!items_in || !codes || !offsets3 || !items_out || !far_keys || !near_keys)) ||
+      (I > 0 && (!left || !right)))
+    return fail(JFM_ERR_ARG, "jfm_tree_walk_emit: bad argument");
+  if (n_in == 0) return JFM_OK;
+  hipLaunchKernelGGL(jfm::walk_emit_kernel, dim3(jfm::grid_for(n_in)), dim3(256), 0, jfm::as_stream(stream),
+                     reinterpret_cast<const int2*>(items_in), n_in, I, left, right, codes, offsets3, far_base,
+                     near_base, reinterpret_cast<int2*>(items_out), far_keys, near_keys);
+  return jfm::check_launch("jfm_tree_walk_emit");
+}
+
+int32_t jfm_scan_scratch_bytes(int64_t n, int64_t* bytes) {
+  if (!bytes || n < 0 || n > INT32_MAX) return fail(JFM_ERR_ARG, "jfm_scan_scratch_bytes: bad argument");
+  size_t b = 0;
+  hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, b, static_cast<const int32_t*>(nullptr),
+                                                  static_cast<int32_t*>(nullptr), static_cast<int>(n));
+  if (e != hipSuccess) return fail(JFM_ERR_HIP, std::string("jfm_scan_scratch_bytes: ") + hipGetErrorString(e));
+  *bytes = static_cast<int64_t>(b);
+  return JFM_OK;
+}
+
+int32_t jfm_scan_i32(void* stream, int64_t n, const int32_t* in, int32_t* out, void* scratch, int64_t scratch_bytes) {
+  if (n < 0 || n > INT32_MAX || (n > 0 && (!in || !out))) return fail(JFM_ERR_ARG, "jfm_scan_i32: bad argument");
+  if (n == 0) return JFM_OK;
+  int64_t need = 0;
+  if (int32_t rc = jfm_scan_scratch_bytes(n, &need)) return rc;
+  if (!scratch || scratch_bytes < need) return fail(JFM_ERR_ARG, "jfm_scan_i32: scratch too small");
+  size_t b = static_cast<size_t>(scratch_bytes);
+  hipError_t e = hipcub::DeviceScan::ExclusiveSum(scratch, b, in, out, static_cast<int>(n), jfm::as_stream(stream));
+  if (e != hipSuccess) return fail(JFM_ERR_HIP, std::string("jfm_scan_i32: ") + hipGetErrorString(e));
+  return jfm::check_launch("jfm_scan_i32");
 }
 
 int32_t jfm_keys_to_csr(void* stream, const uint64_t* sorted_keys, int64_t K, int32_t rows, int32_t* off,

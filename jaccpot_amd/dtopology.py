@@ -188,36 +188,39 @@ def build_topology_device(x: torch.Tensor, m: torch.Tensor, *, leaf_size: int, t
              xs.data_ptr(), radius.data_ptr())
     del xs
 
-    # ---- dual-tree walk, breadth first -----------------------------------------------------
-    items = [_Grow(max(1024, 8 * L), torch.int32, dev, 2), _Grow(max(1024, 8 * L), torch.int32, dev, 2)]
-    items[0].buf[0].zero_()  # the root self item (0, 0)
+    # ---- dual-tree walk, breadth first: classify, scan, emit ------------------------------
+    items = torch.zeros((1, 2), dtype=torch.int32, device=dev)  # the root self item (0, 0)
     far = _Grow(max(1024, 128 * L), torch.int64, dev)
     near = _Grow(max(1024, 320 * L), torch.int64, dev)
-    counts = torch.zeros(3, dtype=torch.int64, device=dev)
     n_in, n_far, n_near = 1, 0, 0
+    scan_bytes = ctypes.c_int64(0)
+    scratch = torch.empty(1, dtype=torch.uint8, device=dev)
     while n_in > 0:
-        src, dst = items
-        while True:
-            counts.copy_(torch.tensor([0, n_far, n_near], dtype=torch.int64), non_blocking=False)
-            nat.call("jfm_tree_walk_step", s, I, src.buf.data_ptr(), n_in, left.data_ptr(), right.data_ptr(),
-                     center.data_ptr(), radius.data_ptr(), float(theta), dst.buf.data_ptr(), dst.cap,
-                     far.buf.data_ptr(), far.cap, near.buf.data_ptr(), near.cap, counts.data_ptr())
-            c = counts.cpu().numpy()
-            ok = True
-            if c[0] > dst.cap:
-                dst.grow(int(c[0]), 0)
-                ok = False
-            if c[1] > far.cap:
-                far.grow(int(c[1]), n_far)
-                ok = False
-            if c[2] > near.cap:
-                near.grow(int(c[2]), n_near)
-                ok = False
-            if ok:
-                break
-        n_in, n_far, n_near = int(c[0]), int(c[1]), int(c[2])
-        items.reverse()
-    del items
+        codes = torch.empty(n_in, dtype=torch.uint8, device=dev)
+        counts = torch.empty((3, n_in), dtype=torch.int32, device=dev)
+        offsets = torch.empty((3, n_in), dtype=torch.int32, device=dev)
+        nat.call("jfm_tree_walk_classify", s, I, items.data_ptr(), n_in, center.data_ptr(), radius.data_ptr(),
+                 float(theta), codes.data_ptr(), counts.data_ptr())
+        nat.call("jfm_scan_scratch_bytes", n_in, ctypes.byref(scan_bytes))
+        if scratch.numel() < scan_bytes.value:
+            scratch = torch.empty(int(scan_bytes.value), dtype=torch.uint8, device=dev)
+        for r in range(3):
+            nat.call("jfm_scan_i32", s, n_in, counts[r].data_ptr(), offsets[r].data_ptr(), scratch.data_ptr(),
+                     int(scratch.numel()))
+        totals = (offsets[:, -1] + counts[:, -1]).cpu().numpy()
+        t_items, t_far, t_near = (int(v) for v in totals)
+        if n_far + t_far > far.cap:
+            far.grow(n_far + t_far, n_far)
+        if n_near + t_near > near.cap:
+            near.grow(n_near + t_near, n_near)
+        nxt = torch.empty((max(t_items, 1), 2), dtype=torch.int32, device=dev)
+        nat.call("jfm_tree_walk_emit", s, I, items.data_ptr(), n_in, left.data_ptr(), right.data_ptr(),
+                 codes.data_ptr(), offsets.data_ptr(), n_far, n_near, nxt.data_ptr(), far.buf.data_ptr(),
+                 near.buf.data_ptr())
+        items, n_in = nxt, t_items
+        n_far += t_far
+        n_near += t_near
+    del items, codes, counts, offsets, scratch
     fk, _ = _sort_u64(far.buf[:n_far], 32 + _key_bits(max(nodes - 1, 1)))
     del far
     far_off, far_src = _csr(fk, nodes)
