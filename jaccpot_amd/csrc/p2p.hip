@@ -102,6 +102,31 @@ __device__ __forceinline__ void fetch_positions(const T* __restrict__ pos4, int 
   }
 }
 
+// Leaf-granular fetch (power-of-two leaves, leaf_size == WT): thread t takes
+// the LOADS consecutive slots first .. first + LOADS - 1, which lie in one
+// source leaf, so one near-list entry and one slot decode serve them all.
+template <int WT>
+__device__ __forceinline__ int fetch_list_leaf(const int32_t* __restrict__ near_list, int nnear,
+                                               const int32_t* __restrict__ fallback, int first) {
+  const int32_t* list = nnear > 0 ? near_list : fallback;
+  const int qi = static_cast<int>(static_cast<unsigned>(first) / WT);
+  return list[min(max(qi - 1, 0), max(nnear - 1, 0))];
+}
+
+template <typename T, int WT, int LOADS>
+__device__ __forceinline__ void fetch_positions_leaf(const T* __restrict__ pos4, int n, int leaf, int total, int first,
+                                                     int v, V4<T> (&out)[LOADS], bool (&valid)[LOADS]) {
+  const int qi = static_cast<int>(static_cast<unsigned>(first) / WT);
+  const int o = static_cast<int>(static_cast<unsigned>(first) % WT);
+  const int g0 = (qi > 0 ? v : leaf) * WT + o;
+#pragma unroll
+  for (int l = 0; l < LOADS; ++l) {
+    const int g = first + l < total ? g0 + l : n;
+    valid[l] = g < n;
+    out[l] = load4(pos4, min(g, n - 1));
+  }
+}
+
 // Work items split the source slots of leaves with long near lists: item k
 // covers slots [s0, s1) of leaf `leaf`; part < 0 -> the item is the leaf's
 // only one and writes the output rows, else it writes the unscaled partial
@@ -129,6 +154,11 @@ __global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos
   constexpr int TILE = sizeof(T) == 4 ? 1024 : 512;
   constexpr int LOADS = TILE / kThreads;
   static_assert(2 * kThreads <= TILE, "reduction buffer must fit in the tile");
+  // REMAP: leaf-granular fetch; slot s of a tile is stored at
+  // (s % LOADS) * kThreads + s / LOADS, so the fetching thread writes its
+  // LOADS slots to LOADS conflict-free rows and partition p still reads one
+  // address per interaction, (p % LOADS) * kThreads + p / LOADS + k * NPART / LOADS.
+  constexpr bool REMAP = FIXED && (NPART % LOADS == 0);
   __shared__ V4<T> tile[TILE];
 
   // XCD-aware order: workgroups are dealt round-robin to the XCDs, so hand
@@ -156,9 +186,17 @@ __global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos
   bool ok[LOADS];
   int lst[LOADS];  // near-list entries of the tile after `next`
   const int tid = threadIdx.x;
-  fetch_list<WT, FIXED, LOADS, kThreads>(leaf_size, near_src + nb, nsl - 1, near_off, start + tid, lst);
-  fetch_positions<T, WT, FIXED, LOADS, kThreads>(pos4, n, leaf_size, leaf, total, start + tid, lst, next, ok);
-  fetch_list<WT, FIXED, LOADS, kThreads>(leaf_size, near_src + nb, nsl - 1, near_off, start + TILE + tid, lst);
+  if constexpr (REMAP) {
+    lst[0] = fetch_list_leaf<WT>(near_src + nb, nsl - 1, near_off, start + LOADS * tid);
+    fetch_positions_leaf<T, WT, LOADS>(pos4, n, leaf, total, start + LOADS * tid, lst[0], next, ok);
+    lst[0] = fetch_list_leaf<WT>(near_src + nb, nsl - 1, near_off, start + TILE + LOADS * tid);
+  } else {
+    fetch_list<WT, FIXED, LOADS, kThreads>(leaf_size, near_src + nb, nsl - 1, near_off, start + tid, lst);
+    fetch_positions<T, WT, FIXED, LOADS, kThreads>(pos4, n, leaf_size, leaf, total, start + tid, lst, next, ok);
+    fetch_list<WT, FIXED, LOADS, kThreads>(leaf_size, near_src + nb, nsl - 1, near_off, start + TILE + tid, lst);
+  }
+  const int rbase = REMAP ? (part % LOADS) * kThreads + part / LOADS : part;
+  constexpr int RSTEP = REMAP ? NPART / LOADS : NPART;
 
   T sx = 0, sy = 0, sz = 0, sp = 0;
   T kx = 0, ky = 0, kz = 0, kp = 0;
@@ -173,15 +211,22 @@ __global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos
     __syncthreads();
     // unconditional (past the end they yield invalid slots): a branch here
     // would make the compiler wait for the loads at its join
-    fetch_positions<T, WT, FIXED, LOADS, kThreads>(pos4, n, leaf_size, leaf, total, base + TILE + tid, lst, next, ok);
-    fetch_list<WT, FIXED, LOADS, kThreads>(leaf_size, near_src + nb, nsl - 1, near_off, base + 2 * TILE + tid, lst);
+    if constexpr (REMAP) {
+      fetch_positions_leaf<T, WT, LOADS>(pos4, n, leaf, total, base + TILE + LOADS * tid, lst[0], next, ok);
+      lst[0] = fetch_list_leaf<WT>(near_src + nb, nsl - 1, near_off, base + 2 * TILE + LOADS * tid);
+    } else {
+      fetch_positions<T, WT, FIXED, LOADS, kThreads>(pos4, n, leaf_size, leaf, total, base + TILE + tid, lst, next,
+                                                     ok);
+      fetch_list<WT, FIXED, LOADS, kThreads>(leaf_size, near_src + nb, nsl - 1, near_off, base + 2 * TILE + tid,
+                                             lst);
+    }
     // Slots past `total` hold zero-mass sources, so every partition runs the
     // same (wave-uniform, scalar-controlled) trip count.
     const int trips = (min(TILE, total - base) + NPART - 1) / NPART;
     T tx = 0, ty = 0, tz = 0, tp = 0;
 #pragma unroll 8
     for (int k = 0; k < trips; ++k) {
-      const V4<T> s = tile[part + k * NPART];
+      const V4<T> s = tile[rbase + k * RSTEP];
       const T dx = s.x - xi.x;
       const T dy = s.y - xi.y;
       const T dz = s.z - xi.z;
