@@ -155,11 +155,14 @@ __global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos
   constexpr int LOADS = TILE / kThreads;
   static_assert(2 * kThreads <= TILE, "reduction buffer must fit in the tile");
   // REMAP: leaf-granular fetch; slot s of a tile is stored at
-  // (s % LOADS) * kThreads + s / LOADS, so the fetching thread writes its
-  // LOADS slots to LOADS conflict-free rows and partition p still reads one
-  // address per interaction, (p % LOADS) * kThreads + p / LOADS + k * NPART / LOADS.
+  // (s % LOADS) * ROW + s / LOADS, so the fetching thread writes its LOADS
+  // slots to LOADS conflict-free rows and partition p still reads one address
+  // per interaction, (p % LOADS) * ROW + p / LOADS + k * NPART / LOADS.  Rows
+  // are padded by two elements so the partitions a wave reads (p and p + 1
+  // land LOADS rows apart otherwise) fall in distinct LDS banks.
   constexpr bool REMAP = FIXED && (NPART % LOADS == 0);
-  __shared__ V4<T> tile[TILE];
+  constexpr int ROW = REMAP ? kThreads + 2 : kThreads;
+  __shared__ V4<T> tile[LOADS * ROW];
 
   // XCD-aware order: workgroups are dealt round-robin to the XCDs, so hand
   // each XCD a contiguous run of leaves (neighbouring leaves share most of
@@ -195,7 +198,7 @@ __global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos
     fetch_positions<T, WT, FIXED, LOADS, kThreads>(pos4, n, leaf_size, leaf, total, start + tid, lst, next, ok);
     fetch_list<WT, FIXED, LOADS, kThreads>(leaf_size, near_src + nb, nsl - 1, near_off, start + TILE + tid, lst);
   }
-  const int rbase = REMAP ? (part % LOADS) * kThreads + part / LOADS : part;
+  const int rbase = REMAP ? (part % LOADS) * ROW + part / LOADS : part;
   constexpr int RSTEP = REMAP ? NPART / LOADS : NPART;
 
   T sx = 0, sy = 0, sz = 0, sp = 0;
@@ -206,7 +209,7 @@ __global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos
     for (int l = 0; l < LOADS; ++l) {
       V4<T> q = next[l];
       if (!ok[l]) q.w = T(0);
-      tile[l * kThreads + threadIdx.x] = q;
+      tile[l * ROW + threadIdx.x] = q;
     }
     __syncthreads();
     // unconditional (past the end they yield invalid slots): a branch here
