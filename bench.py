@@ -105,8 +105,8 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md for how it would "
-              "shard); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: the FMM path here runs on one GPU (no multi-GPU lane); "
+              "run with --gpus 1", file=sys.stderr)
         return 2
 
     import torch

@@ -269,7 +269,7 @@ int32_t jfm_l2l_level(void* stream, int32_t dtype, int32_t p, const int32_t* nod
  * adding once-rounded results to the float32 rows below the shadows.  Call it
  * after jfm_m2l and before the float32 jfm_l2l_level sweeps of depth >= 4.
  * The float64 M2M/L2L of the top levels of jaccpot's large-N lane runs in
- * its working precision; the split is this build's precision policy. */
+ * its working precision; the split is this library's precision policy. */
 int32_t jfm_m2m_top(void* stream, int32_t p, const int32_t* nodes, const int32_t* lev_off, int32_t nlev,
                     const int32_t* lev_off_host, const int32_t* left, const int32_t* right, const int32_t* node_range,
                     const int32_t* shadow, void* multipoles, void* centers4, void* mult64, void* centers64);

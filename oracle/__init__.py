@@ -18,7 +18,7 @@ fmm        -- the full fixed-order FMM (P2M -> M2M -> M2L -> L2L -> L2P + P2P).
 direct     -- the float64 O(N^2) direct-sum oracle.
 cfmm       -- ctypes wrapper of cfmm.cpp, a C++/OpenMP restatement of the
               fmm numerics (float64 or float32, optional per-pair orders) on a
-              given topology: the checker at large N and the CPU baseline.
+              given topology: the oracle at large N and the CPU baseline.
 labels     -- per-pair truncation-error labels of the learned order.
 mlp        -- the order network's forward/backward in NumPy.
 ics        -- seeded initial-condition generators for the benchmark configs.

@@ -1,7 +1,7 @@
 """Seeded initial conditions for the benchmark workloads (oracle side).
 
-The product package has its own generators (``jaccpot_amd.ics``); this copy is
-the checker's.  Both follow the same published formulas:
+The product package has its own generators (``jaccpot_amd.ics``); this copy
+belongs to the oracle.  Both follow the same published formulas:
 
 * uniform: U[-1,1]^3, masses U[0.5,1.5];
 * Plummer a=1 truncated at r=10a: u ~ U(0, M(10)), M(r) = r^3/(r^2+1)^{3/2},

@@ -1,6 +1,6 @@
 """Float64 oracle of the per-pair truncation-error labels (test infrastructure only).
 
-The labels are this build's own training targets for the learned M2L order
+The labels are this library's own training targets for the learned M2L order
 (the reference has no counterpart).  For a far pair (target t, source s) with
 displacement d = c_t - c_s, the order-8 translation L8 = M2L_8(M_s) is the
 reference and the gear-g translation Lg = M2L_g(M_s truncated to degree g) is

@@ -1,6 +1,6 @@
 """The C++/OpenMP FMM restatement (oracle/cfmm.cpp) against the NumPy oracle.
 
-It is the checker of the GPU path at sizes the NumPy pipeline cannot reach
+It is the oracle of the GPU path at sizes the NumPy pipeline cannot reach
 and the benchmark's CPU baseline, so it is pinned here: on the same topology
 it must reproduce oracle.fmm in float64 to rounding (accelerations and
 potentials, every order, per-pair orders), while a change of order moves the
