@@ -305,14 +305,12 @@ int32_t jfm_p2p(void* stream, int32_t dtype, const void* pos4, int32_t n, int32_
  * targets' unscaled partial sums to partials[(item_part[k] * leaf_size + i) * 4
  * + {0..3}] (num_part rows of room).  jfm_p2p_reduce then adds, for every
  * split leaf, the partials split_p0[k] .. split_p1[k]-1 in that order
- * (compensated) and writes or adds the output rows.  max_blocks > 0 caps the
- * workgroups of the launch (each then takes several items in turn): the
- * P2P's share of the GPU while another stream runs beside it. */
+ * (compensated) and writes or adds the output rows. */
 int32_t jfm_p2p_items(void* stream, int32_t dtype, const void* pos4, int32_t n, int32_t num_leaves, int32_t leaf_size,
                       int32_t num_items, const int32_t* item_leaf, const int32_t* item_s0, const int32_t* item_s1,
                       const int32_t* item_part, int32_t num_part, void* partials, const int32_t* near_off,
                       const int32_t* near_src, double eps2, double G, const int32_t* out_index, void* acc, void* pot,
-                      int32_t flags, int32_t accumulate, int32_t max_blocks);
+                      int32_t flags, int32_t accumulate);
 int32_t jfm_p2p_reduce(void* stream, int32_t dtype, const void* pos4, int32_t n, int32_t leaf_size, int32_t num_split,
                        const int32_t* split_leaf, const int32_t* split_p0, const int32_t* split_p1,
                        const void* partials, double eps2, double G, const int32_t* out_index, void* acc, void* pot,

@@ -141,7 +141,6 @@ struct P2PItems {
   const int32_t* part;
   T* partials;
   int32_t num_part;  // rows of partials (items with part >= num_part write nothing)
-  int32_t num_items;  // items of the launch (one per leaf when leaf is NULL)
 };
 
 // SAFE: guard r^2 == 0 (unsoftened self term / coincident points).
@@ -169,11 +168,7 @@ __global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos
   // each XCD a contiguous run of leaves (neighbouring leaves share most of
   // their source leaves, which then stay in that XCD's L2).  Bijective:
   // XCD x < r owns q+1 leaves, the others q, where nblk = 8q + r.
-  // A launch may have fewer workgroups than items (a cap on the P2P's share of
-  // the GPU while other streams run): workgroup b then takes items b, b + G, ...
-  // of the XCD-contiguous order (G a multiple of 8 keeps each XCD's items its own).
-  for (int it = blockIdx.x; it < items.num_items; it += gridDim.x) {
-  const int item = xcd_contiguous(it, items.num_items);
+  const int item = xcd_contiguous(blockIdx.x, gridDim.x);
   const int leaf = items.leaf ? items.leaf[item] : item;
   const int i = threadIdx.x % WT;
   const int part = threadIdx.x / WT;
@@ -277,30 +272,27 @@ __global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos
     const V4<T> r = tile[i];
     T ax = r.x, ay = r.y, az = r.z, ap = r.w;
     if (part_out >= 0) {
-      if (part_out < items.num_part) {
-        T* dst = items.partials + (static_cast<int64_t>(part_out) * leaf_size + i) * 4;
-        dst[0] = ax;
-        dst[1] = ay;
-        dst[2] = az;
-        dst[3] = ap;
-      }
-    } else {
-      if (POT && eps2 > T(0)) ap -= xi.w * fast_rsqrt(eps2);  // remove the softened self term
-      const int64_t row = out_index ? out_index[first + i] : first + i;
-      if (accumulate) {
-        acc[3 * row + 0] += G * ax;
-        acc[3 * row + 1] += G * ay;
-        acc[3 * row + 2] += G * az;
-        if (POT) pot[row] += -G * ap;
-      } else {
-        acc[3 * row + 0] = G * ax;
-        acc[3 * row + 1] = G * ay;
-        acc[3 * row + 2] = G * az;
-        if (POT) pot[row] = -G * ap;
-      }
+      if (part_out >= items.num_part) return;
+      T* dst = items.partials + (static_cast<int64_t>(part_out) * leaf_size + i) * 4;
+      dst[0] = ax;
+      dst[1] = ay;
+      dst[2] = az;
+      dst[3] = ap;
+      return;
     }
-  }
-  __syncthreads();  // the next item reuses the tile
+    if (POT && eps2 > T(0)) ap -= xi.w * fast_rsqrt(eps2);  // remove the softened self term
+    const int64_t row = out_index ? out_index[first + i] : first + i;
+    if (accumulate) {
+      acc[3 * row + 0] += G * ax;
+      acc[3 * row + 1] += G * ay;
+      acc[3 * row + 2] += G * az;
+      if (POT) pot[row] += -G * ap;
+    } else {
+      acc[3 * row + 0] = G * ax;
+      acc[3 * row + 1] = G * ay;
+      acc[3 * row + 2] = G * az;
+      if (POT) pot[row] = -G * ap;
+    }
   }
 }
 
@@ -389,9 +381,7 @@ int32_t launch_p2p_wt(hipStream_t s, const T* pos4, int32_t n, int32_t leaf_size
 template <typename T>
 int32_t launch_p2p(hipStream_t s, const T* pos4, int32_t n, int32_t leaf_size, int32_t num_leaves,
                    const int32_t* near_off, const int32_t* near_src, T eps2, T G, const int32_t* out_index, T* acc,
-                   T* pot, bool comp, bool accumulate, P2PItems<T> items = P2PItems<T>{}, int32_t max_blocks = 0) {
-  if (!items.leaf) items.num_items = num_leaves;
-  if (max_blocks > 0 && max_blocks < items.num_items) num_leaves = max_blocks;  // grid of the launch
+                   T* pot, bool comp, bool accumulate, P2PItems<T> items = P2PItems<T>{}) {
   if (leaf_size <= 8)
     return launch_p2p_wt<T, 8>(s, pos4, n, leaf_size, num_leaves, near_off, near_src, eps2, G, out_index, acc,
                                pot, comp, accumulate, items);
@@ -447,7 +437,7 @@ int32_t jfm_p2p_items(void* stream, int32_t dtype, const void* pos4, int32_t n, 
                       int32_t num_items, const int32_t* item_leaf, const int32_t* item_s0, const int32_t* item_s1,
                       const int32_t* item_part, int32_t num_part, void* partials, const int32_t* near_off,
                       const int32_t* near_src, double eps2, double G, const int32_t* out_index, void* acc, void* pot,
-                      int32_t flags, int32_t accumulate, int32_t max_blocks) {
+                      int32_t flags, int32_t accumulate) {
   if (n < 1 || leaf_size < 1 || leaf_size > 256 || num_leaves != (n + leaf_size - 1) / leaf_size || num_items < 1 ||
       num_part < 0)
     return fail(JFM_ERR_ARG, "jfm_p2p_items: inconsistent sizes");
@@ -463,15 +453,11 @@ int32_t jfm_p2p_items(void* stream, int32_t dtype, const void* pos4, int32_t n, 
     return launch_p2p<float>(s, (const float*)pos4, n, leaf_size, num_items, near_off, near_src,
                              static_cast<float>(eps2), static_cast<float>(G), out_index, (float*)acc, (float*)potp,
                              comp, accumulate != 0,
-                             P2PItems<float>{item_leaf, item_s0, item_s1, item_part, (float*)partials, num_part,
-                                             num_items},
-                             max_blocks);
+                             P2PItems<float>{item_leaf, item_s0, item_s1, item_part, (float*)partials, num_part});
   if (dtype == JFM_F64)
     return launch_p2p<double>(s, (const double*)pos4, n, leaf_size, num_items, near_off, near_src, eps2, G,
                               out_index, (double*)acc, (double*)potp, comp, accumulate != 0,
-                              P2PItems<double>{item_leaf, item_s0, item_s1, item_part, (double*)partials, num_part,
-                                               num_items},
-                              max_blocks);
+                              P2PItems<double>{item_leaf, item_s0, item_s1, item_part, (double*)partials, num_part});
   return fail(JFM_ERR_ARG, "jfm_p2p_items: bad dtype");
 }
 

@@ -520,7 +520,7 @@ class FastMultipoleMethod:
                  topo.leaf_size, items.num_items, items.leaf.data_ptr(), items.s0.data_ptr(), items.s1.data_ptr(),
                  items.part.data_ptr(), items.num_part, items.partials.data_ptr(), state.near_off.data_ptr(),
                  state.near_src.data_ptr(), state.softening ** 2, state.G, state.perm.data_ptr(), acc.data_ptr(),
-                 potp, flags, 1 if accumulate else 0, int(getattr(items, "max_blocks", 0)))
+                 potp, flags, 1 if accumulate else 0)
         nat.call("jfm_p2p_reduce", s, code, state.pos4.data_ptr(), topo.num_particles, topo.leaf_size,
                  items.num_split, items.split_leaf.data_ptr(), items.split_p0.data_ptr(), items.split_p1.data_ptr(),
                  items.partials.data_ptr(), state.softening ** 2, state.G, state.perm.data_ptr(), acc.data_ptr(),
