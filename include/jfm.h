@@ -299,7 +299,8 @@ int32_t jfm_p2p(void* stream, int32_t dtype, const void* pos4, int32_t n, int32_
  * near-field launches, runtime/kernels/nearfield/_kernels.py:40-338, cut
  * into equal slot ranges so no leaf's sources serialise on one workgroup).
  * Item k covers slots [item_s0[k], item_s1[k]) of leaf item_leaf[k]'s
- * sequence (self leaf, then its near leaves, leaf_size slots each);
+ * sequence (self leaf, then its near leaves, leaf_size slots each); item_s0
+ * and item_s1 are multiples of leaf_size (whole source leaves);
  * item_part[k] < 0: the item is its leaf's only one and writes (or adds to,
  * accumulate != 0) the output rows like jfm_p2p; otherwise it writes its
  * targets' unscaled partial sums to partials[(item_part[k] * leaf_size + i) * 4

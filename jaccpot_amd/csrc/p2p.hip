@@ -27,6 +27,16 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kXcds = 8;
+// LDS per CU is 160 KiB; a workgroup holding just over 1/(N+1) of it caps the
+// CU at N workgroups (LDS is allocated in 512-byte granules).
+constexpr int kLdsPerCu = 160 * 1024;
+#ifndef JFM_P2P_BLOCKS_PER_CU
+#define JFM_P2P_BLOCKS_PER_CU 5
+#endif
+constexpr int kP2PBlocksPerCu = JFM_P2P_BLOCKS_PER_CU;
+constexpr int kP2PLdsBytes = (kLdsPerCu / (kP2PBlocksPerCu + 1) / 512 + 1) * 512;
+static_assert(kP2PLdsBytes * (kP2PBlocksPerCu + 1) > kLdsPerCu && kP2PLdsBytes * kP2PBlocksPerCu <= kLdsPerCu,
+              "P2P LDS reservation must cap the CU at kP2PBlocksPerCu workgroups");
 
 __device__ __forceinline__ int xcd_contiguous(int b, int nblk) {
   const int q = nblk / kXcds, r = nblk % kXcds;
@@ -127,6 +137,32 @@ __device__ __forceinline__ void fetch_positions_leaf(const T* __restrict__ pos4,
   }
 }
 
+// float32 leaf-granular fetch through a raw buffer resource over the n rows
+// of pos4 (16 bytes each).  The hardware range check returns zero for a row
+// at or past n, i.e. a zero-mass source at the origin, so the missing rows of
+// a ragged last leaf and slots past `total` (sent to row n) need no clamp and
+// no mask, and the LOADS rows of a thread share one offset register (the row
+// steps go into the instructions' immediate offsets).  Requires total and
+// first to be multiples of LOADS (slot ranges are whole leaves) and
+// 16 * (n + LOADS) < 2^31.
+constexpr int kBufMaxRows = (1 << 27) - 64;
+constexpr int kBufDword3 = 0x00020000;  // untyped 32-bit data format
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t pos4_rsrc(const void* pos4, int n) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(pos4), static_cast<short>(0), n * 16, kBufDword3);
+}
+
+template <int WT, int LOADS>
+__device__ __forceinline__ void fetch_positions_buf(__amdgpu_buffer_rsrc_t rsrc, int n, int leaf, int total,
+                                                    int first, int v, V4<float> (&out)[LOADS]) {
+  const int qi = static_cast<int>(static_cast<unsigned>(first) / WT);
+  const int o = static_cast<int>(static_cast<unsigned>(first) % WT);
+  const int g0 = first < total ? (qi > 0 ? v : leaf) * WT + o : n;
+#pragma unroll
+  for (int l = 0; l < LOADS; ++l)
+    out[l] = __builtin_bit_cast(V4<float>, __builtin_amdgcn_raw_buffer_load_b128(rsrc, g0 * 16 + 16 * l, 0, 0));
+}
+
 // Work items split the source slots of leaves with long near lists: item k
 // covers slots [s0, s1) of leaf `leaf`; part < 0 -> the item is the leaf's
 // only one and writes the output rows, else it writes the unscaled partial
@@ -161,8 +197,15 @@ __global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos
   // are padded by two elements so the partitions a wave reads (p and p + 1
   // land LOADS rows apart otherwise) fall in distinct LDS banks.
   constexpr bool REMAP = FIXED && (NPART % LOADS == 0);
+  constexpr bool BUF = REMAP && sizeof(T) == 4;  // float32: buffer-resource fetch
   constexpr int ROW = REMAP ? kThreads + 2 : kThreads;
-  __shared__ V4<T> tile[LOADS * ROW];
+  // The tile array is padded to kP2PLdsBytes so that at most kP2PBlocksPerCu
+  // workgroups share a CU: the rest of its registers and LDS stay free for the
+  // tree-pass kernels that run beside P2P in an overlapped step (at full
+  // occupancy their small dependent launches wait for P2P workgroups to end).
+  constexpr int kTileElems = LOADS * ROW;
+  constexpr int kPadElems = static_cast<int>(kP2PLdsBytes / sizeof(V4<T>));
+  __shared__ V4<T> tile[kTileElems > kPadElems ? kTileElems : kPadElems];
 
   // XCD-aware order: workgroups are dealt round-robin to the XCDs, so hand
   // each XCD a contiguous run of leaves (neighbouring leaves share most of
@@ -189,9 +232,14 @@ __global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos
   bool ok[LOADS];
   int lst[LOADS];  // near-list entries of the tile after `next`
   const int tid = threadIdx.x;
+  const __amdgpu_buffer_rsrc_t rsrc = pos4_rsrc(pos4, BUF ? n : 0);  // used by the float32 fetch only
   if constexpr (REMAP) {
     lst[0] = fetch_list_leaf<WT>(near_src + nb, nsl - 1, near_off, start + LOADS * tid);
-    fetch_positions_leaf<T, WT, LOADS>(pos4, n, leaf, total, start + LOADS * tid, lst[0], next, ok);
+    if constexpr (BUF)
+      fetch_positions_buf<WT, LOADS>(rsrc, n, leaf, total, start + LOADS * tid, lst[0],
+                                     reinterpret_cast<V4<float>(&)[LOADS]>(next));
+    else
+      fetch_positions_leaf<T, WT, LOADS>(pos4, n, leaf, total, start + LOADS * tid, lst[0], next, ok);
     lst[0] = fetch_list_leaf<WT>(near_src + nb, nsl - 1, near_off, start + TILE + LOADS * tid);
   } else {
     fetch_list<WT, FIXED, LOADS, kThreads>(leaf_size, near_src + nb, nsl - 1, near_off, start + tid, lst);
@@ -208,13 +256,17 @@ __global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos
 #pragma unroll
     for (int l = 0; l < LOADS; ++l) {
       V4<T> q = next[l];
-      if (!ok[l]) q.w = T(0);
+      if (!BUF && !ok[l]) q.w = T(0);
       tile[l * ROW + threadIdx.x] = q;
     }
     __syncthreads();
     // unconditional (past the end they yield invalid slots): a branch here
     // would make the compiler wait for the loads at its join
-    if constexpr (REMAP) {
+    if constexpr (BUF) {
+      fetch_positions_buf<WT, LOADS>(rsrc, n, leaf, total, base + TILE + LOADS * tid, lst[0],
+                                     reinterpret_cast<V4<float>(&)[LOADS]>(next));
+      lst[0] = fetch_list_leaf<WT>(near_src + nb, nsl - 1, near_off, base + 2 * TILE + LOADS * tid);
+    } else if constexpr (REMAP) {
       fetch_positions_leaf<T, WT, LOADS>(pos4, n, leaf, total, base + TILE + LOADS * tid, lst[0], next, ok);
       lst[0] = fetch_list_leaf<WT>(near_src + nb, nsl - 1, near_off, base + 2 * TILE + LOADS * tid);
     } else {
@@ -355,7 +407,7 @@ int32_t launch_p2p_wt(hipStream_t s, const T* pos4, int32_t n, int32_t leaf_size
   // the common case (potential off, softened, compensated, leaf_size == WT) gets the
   // compile-time slot decode; every other combination the general one
 #define JFM_P2P_LAUNCH(POT, COMP, SAFE)                                                                      \
-  if (!POT && COMP && !SAFE && leaf_size == WT)                                                              \
+  if (!POT && COMP && !SAFE && leaf_size == WT && (sizeof(T) == 8 || n <= kBufMaxRows))                      \
     p2p_kernel<T, WT, POT, COMP, SAFE, true><<<num_blocks, kThreads, 0, s>>>(                                 \
         pos4, n, leaf_size, near_off, near_src, eps2, G, out_index, acc, pot, accumulate ? 1 : 0, items);    \
   else                                                                                                       \

@@ -40,6 +40,11 @@ class P2PItems:
 def build_p2p_items(near_off: np.ndarray, leaf_size: int, dtype: torch.dtype, device,
                     chunk_slots: int = CHUNK_SLOTS):
     """Items for the near CSR ``near_off`` [L+1]; None when every leaf fits one item."""
+    if chunk_slots < 1:
+        raise ValueError(f"chunk_slots must be positive, got {chunk_slots}")
+    # item bounds are whole source leaves (the leaf-granular fetch of the P2P
+    # kernel reads a leaf's slots in aligned groups)
+    chunk_slots = max(chunk_slots // int(leaf_size), 1) * int(leaf_size)
     near_off = np.asarray(near_off, np.int64)
     L = near_off.size - 1
     slots = (np.diff(near_off) + 1) * int(leaf_size)

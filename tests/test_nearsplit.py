@@ -37,6 +37,26 @@ def test_items_cover_every_slot_once():
     assert it.partials.shape == (it.num_part, leaf, 4)
 
 
+@pytest.mark.parametrize("leaf,chunk", [(5, 64), (12, 100), (8, 20), (3, 2)])
+def test_item_bounds_are_whole_leaves(leaf, chunk):
+    """The chunk is rounded down to whole source leaves (at least one)."""
+    from jaccpot_amd.nearsplit import build_p2p_items
+
+    rng = np.random.default_rng(1)
+    counts = rng.integers(0, 60, size=200)
+    near_off = np.concatenate([[0], np.cumsum(counts)])
+    it = build_p2p_items(near_off, leaf, torch.float32, "cpu", chunk_slots=chunk)
+    s0, s1, leaf_of = it.s0.numpy(), it.s1.numpy(), it.leaf.numpy()
+    step = max(chunk // leaf, 1) * leaf
+    assert np.all(s0 % leaf == 0) and np.all(s1 % leaf == 0)
+    assert np.all(s1 - s0 <= step) and np.all(s1 > s0)
+    # the items of each leaf tile its whole slot sequence in order
+    for l in (0, 57, 199):
+        sel = np.nonzero(leaf_of == l)[0]
+        assert s0[sel[0]] == 0 and s1[sel[-1]] == (counts[l] + 1) * leaf
+        assert np.all(s1[sel[:-1]] == s0[sel[1:]])
+
+
 def test_no_items_when_every_leaf_fits():
     from jaccpot_amd.nearsplit import build_p2p_items
 
