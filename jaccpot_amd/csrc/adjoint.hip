@@ -161,6 +161,14 @@ __device__ __forceinline__ void weighted_harmonics(T dx, T dy, T dz, T w, T bx, 
 // p2p_adjoint_reduce_kernel adds them in item order.  leaf == NULL: one item
 // per leaf.
 template <typename T>
+__device__ __forceinline__ T fast_rsqrt(T x) {
+  if constexpr (sizeof(T) == 4)
+    return __builtin_amdgcn_rsqf(x);
+  else
+    return T(1) / dev_sqrt(x);
+}
+
+template <typename T>
 struct AdjItems {
   const int32_t* leaf;
   const int32_t* s0;
@@ -213,22 +221,30 @@ __global__ __launch_bounds__(kThreads) void p2p_adjoint_kernel(const T* __restri
     }
     __syncthreads();
     const int trips = (min(TILE, total - base) + NPART - 1) / NPART;
+    // per pair: xbar += s3 u - (3 s5 (d.u) + s3 pw) d,  mbar -= s3 (d.abar_j) + s phib_j
+    // (the radial terms share one factor; s = 1/r from the hardware
+    // reciprocal square root in float32, as in the forward P2P)
+#pragma unroll 4
     for (int k = 0; k < trips; ++k) {
       const V4<T> s = tx[part + k * NPART];
       const V4<T> c = tc[part + k * NPART];
       const T dx = s.x - xi.x, dy = s.y - xi.y, dz = s.z - xi.z;
-      const T r2 = dx * dx + dy * dy + dz * dz + eps2;
-      const T sv = SAFE ? (r2 > T(0) ? T(1) / dev_sqrt(r2) : T(0)) : T(1) / dev_sqrt(r2);
-      const T s3 = sv * sv * sv;
-      const T s5 = s3 * sv * sv;
+      const T r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, eps2)));
+      T sv = fast_rsqrt(r2);
+      if (SAFE) sv = r2 > T(0) ? sv : T(0);
+      const T sv2 = sv * sv;
+      const T s3 = sv2 * sv;
+      const T s5 = s3 * sv2;
       // u = m_i abar_j - m_j abar_i
-      const T ux = xi.w * c.x - s.w * ci.x, uy = xi.w * c.y - s.w * ci.y, uz = xi.w * c.z - s.w * ci.z;
-      const T du = dx * ux + dy * uy + dz * uz;
-      const T pw = s.w * ci.w + xi.w * c.w;
-      gx += s3 * ux - T(3) * s5 * du * dx - s3 * pw * dx;
-      gy += s3 * uy - T(3) * s5 * du * dy - s3 * pw * dy;
-      gz += s3 * uz - T(3) * s5 * du * dz - s3 * pw * dz;
-      gm += -s3 * (dx * c.x + dy * c.y + dz * c.z) - sv * c.w;
+      const T ux = fma(xi.w, c.x, -s.w * ci.x), uy = fma(xi.w, c.y, -s.w * ci.y), uz = fma(xi.w, c.z, -s.w * ci.z);
+      const T du = fma(dx, ux, fma(dy, uy, dz * uz));
+      const T pw = fma(s.w, ci.w, xi.w * c.w);
+      const T a = fma(T(3) * s5, du, s3 * pw);
+      gx = fma(s3, ux, fma(-a, dx, gx));
+      gy = fma(s3, uy, fma(-a, dy, gy));
+      gz = fma(s3, uz, fma(-a, dz, gz));
+      const T dc = fma(dx, c.x, fma(dy, c.y, dz * c.z));
+      gm = fma(-s3, dc, fma(-sv, c.w, gm));
     }
   }
   __syncthreads();
