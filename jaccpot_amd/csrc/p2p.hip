@@ -404,10 +404,10 @@ int32_t launch_p2p_wt(hipStream_t s, const T* pos4, int32_t n, int32_t leaf_size
                       T* acc, T* pot, bool comp, bool accumulate, P2PItems<T> items) {
   const bool want_pot = pot != nullptr;
   const bool safe = !(eps2 > T(0));
-  // the common case (potential off, softened, compensated, leaf_size == WT) gets the
-  // compile-time slot decode; every other combination the general one
+  // the common cases (softened, compensated, leaf_size == WT) get the compile-time
+  // slot decode and leaf-granular fetch; every other combination the general one
 #define JFM_P2P_LAUNCH(POT, COMP, SAFE)                                                                      \
-  if (!POT && COMP && !SAFE && leaf_size == WT && (sizeof(T) == 8 || n <= kBufMaxRows))                      \
+  if (COMP && !SAFE && leaf_size == WT && (sizeof(T) == 8 || n <= kBufMaxRows))                              \
     p2p_kernel<T, WT, POT, COMP, SAFE, true><<<num_blocks, kThreads, 0, s>>>(                                 \
         pos4, n, leaf_size, near_off, near_src, eps2, G, out_index, acc, pot, accumulate ? 1 : 0, items);    \
   else                                                                                                       \
