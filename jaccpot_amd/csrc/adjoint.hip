@@ -14,7 +14,8 @@
 //              (forward M2L on flipped rows, m2l is symmetric);
 //   centres:   cbar_t = Gr(L_t^own) . Lbar_t - Gr(Mbar_t^own) . M_t
 //              + sum_s d/dd [Lbar_t . m2l(M_s, d)] - sum_s d/dd [Lbar_s . m2l(M_t, d)]
-//              (this file: center_adjoint_kernel, dual numbers for d/dd);
+//              (this file: center_adjoint_kernel; d/dd [A . m2l_p(B, d)] =
+//              -A . Gr(m2l_{p+1}(B, d)), exact and valid on the z axis);
 //   down:      Mbar_child += m2m^T(Mbar_parent) = l2l(...) (forward L2L, flipped);
 //   leaves:    x_j += m_j grad(Mbar . U)(x_j - c), m_j += Mbar . U(x_j - c)
 //              (P2M), plus the centre-of-mass chain of every ancestor.
@@ -29,7 +30,7 @@
 #include "launch.hpp"
 #include "rows.hpp"
 
-// Orders with adjoint kernels (the dual-number centre pass is large at p > 6).
+// Orders with adjoint kernels (p <= 6; the centre pass translates at order p + 1).
 #define JFM_FOR_ADJ_ORDER(p, ...)                                                      \
   switch (p) {                                                                         \
     case 1: { constexpr int P = 1; __VA_ARGS__; } break;                               \
@@ -43,46 +44,6 @@
   }
 
 namespace jfm {
-// Forward-mode dual number (value, one directional derivative).  Kept in its
-// own namespace so its fma/sqrt overloads are found by argument-dependent
-// lookup only and never hide the scalar ones.
-namespace ad {
-template <typename T>
-struct Dual {
-  T v, d;
-  __host__ __device__ Dual() : v(0), d(0) {}
-  __host__ __device__ Dual(double x) : v(static_cast<T>(x)), d(0) {}  // NOLINT: constants lift implicitly
-  __host__ __device__ Dual(T x, T dx) : v(x), d(dx) {}
-};
-template <typename T>
-__host__ __device__ inline Dual<T> operator+(Dual<T> a, Dual<T> b) { return {a.v + b.v, a.d + b.d}; }
-template <typename T>
-__host__ __device__ inline Dual<T> operator-(Dual<T> a, Dual<T> b) { return {a.v - b.v, a.d - b.d}; }
-template <typename T>
-__host__ __device__ inline Dual<T> operator-(Dual<T> a) { return {-a.v, -a.d}; }
-template <typename T>
-__host__ __device__ inline Dual<T> operator*(Dual<T> a, Dual<T> b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
-template <typename T>
-__host__ __device__ inline Dual<T> operator/(Dual<T> a, Dual<T> b) {
-  const T q = a.v / b.v;
-  return {q, (a.d - q * b.d) / b.v};
-}
-template <typename T>
-__host__ __device__ inline Dual<T>& operator+=(Dual<T>& a, Dual<T> b) { return a = a + b; }
-template <typename T>
-__host__ __device__ inline bool operator>(Dual<T> a, Dual<T> b) { return a.v > b.v; }
-template <typename T>
-__host__ __device__ inline Dual<T> fma(Dual<T> a, Dual<T> b, Dual<T> c) {
-  return {static_cast<T>(::fma(a.v, b.v, c.v)), a.d * b.v + a.v * b.d + c.d};
-}
-template <typename T>
-__host__ __device__ inline Dual<T> sqrt(Dual<T> a) {
-  const T s = ::sqrt(a.v);
-  return {s, a.d / (T(2) * s)};
-}
-}  // namespace ad
-using ad::Dual;
-
 namespace {
 
 constexpr int kWave = 64;
@@ -372,19 +333,31 @@ __global__ __launch_bounds__(kWave) void leaf_adjoint_a_kernel(const T* __restri
 // --------------------------------------------------------- centre cotangents
 // d/dd [ A . m2l(B, d) ] along `axis` by a dual-number pass of the forward cascade.
 template <typename T, int P>
-__device__ __forceinline__ T m2l_bilinear_ddelta(const T* A, const T* B, T dx, T dy, T dz, int axis) {
-  constexpr int C = ncoef(P);
-  using D = Dual<T>;
-  D v[C], w[C];
-  for (int j = 0; j < C; ++j) v[j] = D(B[j], T(0));
-  Align<D, P> a;
-  a.init(D(dx, axis == 0 ? T(1) : T(0)), D(dy, axis == 1 ? T(1) : T(0)), D(dz, axis == 2 ? T(1) : T(0)));
-  rotate_all<D, P, Rot::ToZMultipole>(v, a);
-  zm2l<D, P>(v, w, a.r);
-  rotate_all<D, P, Rot::FromZLocal>(w, a);
-  T s = T(0);
-  for (int j = 0; j < C; ++j) s = fma(A[j], w[j].d, s);
-  return s;
+__device__ __forceinline__ void m2l_bilinear_grad(const T* A, const T* B, T dx, T dy, T dz, T& gx, T& gy, T& gz) {
+  // d/dd [A . m2l_P(B, d)] = -A . Gr(m2l_{P+1}(B, d)), Gr the local-gradient
+  // map: the separation derivative of a truncated translation is, term by
+  // term, the gradient of the translation one order higher.  No derivative of
+  // the alignment angles is involved, so it also holds for separations on the
+  // z axis, where the angles are fixed by convention.
+  constexpr int C = ncoef(P), C1 = ncoef(P + 1);
+  T v[C1], w[C1];
+  for (int j = 0; j < C1; ++j) v[j] = j < C ? B[j] : T(0);
+  Align<T, P + 1> a;
+  a.init(dx, dy, dz);
+  rotate_all<T, P + 1, Rot::ToZMultipole>(v, a);
+  zm2l<T, P + 1>(v, w, a.r);
+  rotate_all<T, P + 1, Rot::FromZLocal>(w, a);
+  T Gx[C], Gy[C], Gz[C];
+  grad_coeffs<T, P + 1>(w, Gx, Gy, Gz);
+  T sx = T(0), sy = T(0), sz = T(0);
+  for (int j = 0; j < C; ++j) {
+    sx = fma(A[j], Gx[j], sx);
+    sy = fma(A[j], Gy[j], sy);
+    sz = fma(A[j], Gz[j], sz);
+  }
+  gx = -sx;
+  gy = -sy;
+  gz = -sz;
 }
 
 // One wave per node u with far partners: lanes over partners v, summing
@@ -423,9 +396,12 @@ __global__ __launch_bounds__(kWave) void center_adjoint_kernel(
       if (deg & 1) Lv[j] = -Lv[j];
     }
     const T dx = cu.x - cv.x, dy = cu.y - cv.y, dz = cu.z - cv.z;
-    gx += m2l_bilinear_ddelta<T, P>(Lu, Mv, dx, dy, dz, 0) - m2l_bilinear_ddelta<T, P>(Lv, Mu, -dx, -dy, -dz, 0);
-    gy += m2l_bilinear_ddelta<T, P>(Lu, Mv, dx, dy, dz, 1) - m2l_bilinear_ddelta<T, P>(Lv, Mu, -dx, -dy, -dz, 1);
-    gz += m2l_bilinear_ddelta<T, P>(Lu, Mv, dx, dy, dz, 2) - m2l_bilinear_ddelta<T, P>(Lv, Mu, -dx, -dy, -dz, 2);
+    T ax, ay, az, bx, by, bz;
+    m2l_bilinear_grad<T, P>(Lu, Mv, dx, dy, dz, ax, ay, az);
+    m2l_bilinear_grad<T, P>(Lv, Mu, -dx, -dy, -dz, bx, by, bz);
+    gx += ax - bx;
+    gy += ay - by;
+    gz += az - bz;
   }
   red[lane][0] = gx;
   red[lane][1] = gy;

@@ -41,7 +41,19 @@ CASES = [
     ("plummer1200_p4", "plummer", 1200, 16, 4, 0.6, 1e-2),
     ("uniform2000_p3_eps0", "uniform", 2000, 8, 3, 0.5, 0.0),
     ("blobs1500_p6", "blobs", 1500, 16, 6, 0.7, 1e-2),
+    # equal masses on a jittered-free lattice: many node centres share x and y
+    # (far-pair separations along the z axis, where the M2L alignment angle is
+    # fixed by convention and has no derivative)
+    ("lattice1728_p4", "lattice", 1728, 8, 4, 0.6, 1e-2),
 ]
+
+
+def _lattice(n, seed):
+    k = round(n ** (1 / 3))
+    assert k ** 3 == n
+    g = (np.arange(k) + 0.5) / k
+    x = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    return x, np.full(n, 1.0 / n)
 
 
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
@@ -50,7 +62,7 @@ def test_fp64_gradient_matches_central_differences(case, require_gpu):
 
     name, gen, n, leaf, p, theta, eps = case
     print(f"[gpu] adjoint fd {name}", flush=True)
-    x, m = getattr(ics, gen)(n, 41)
+    x, m = _lattice(n, 41) if gen == "lattice" else getattr(ics, gen)(n, 41)
     rng = np.random.default_rng(7)
     w = rng.normal(size=(n, 3))
     u = rng.normal(size=n)
