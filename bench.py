@@ -88,9 +88,10 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="plummer4m_p6", choices=sorted(WORKLOADS))
-    ap.add_argument("--mode", default="learned", choices=("learned", "plain", "learned-only"),
+    ap.add_argument("--mode", default="learned", choices=("learned", "plain", "learned-only", "grad-only"),
                     help="learned: time the learned+mixed FMM (value) and the plain one beside it; "
-                         "learned-only: the learned+mixed FMM alone (profiling)")
+                         "learned-only: the learned+mixed FMM alone (profiling); "
+                         "grad-only: the differentiable forward + backward alone (profiling; prints its object)")
     ap.add_argument("--graph", action="store_true",
                     help="also time the step replayed from a captured HIP graph and report that as ms_per_step")
     ap.add_argument("--no-grad", action="store_true", help="skip the differentiable-path timing")
@@ -115,6 +116,10 @@ def main() -> int:
     from jaccpot_amd import _native as nat
     from jaccpot_amd import ics
     from jaccpot_amd.perf import m2l_flops
+
+    if args.mode == "grad-only":
+        print(json.dumps({"differentiable": measure_gradient(torch, FastMultipoleMethod, ics, args)}), flush=True)
+        return 0
 
     if args.calibrate_copy_mib > 0:
         nel = args.calibrate_copy_mib * (1 << 20) // 4

@@ -176,15 +176,19 @@ __global__ __launch_bounds__(4 * kWave) void m2l_fixup_kernel(int32_t num_span, 
   const int w0 = span_w0[sp], w1 = span_w1[sp];
   for (int j = lane; j < C; j += kWave) {
     T s = scratch[static_cast<int64_t>(w0) * 2 * C + C + j];
-    // block partials loaded eight at a time (one memory round trip per eight
-    // blocks of a long span) and added in block order
+    // block partials added in block order; for rows of one wave pass (the
+    // low gears, whose targets span many blocks) loaded eight at a time, one
+    // memory round trip per eight blocks (the order-8 spans are short and
+    // the unrolled loop costs that kernel occupancy)
     int w = w0 + 1;
-    for (; w + 7 <= w1; w += 8) {
-      T v[8];
+    if constexpr (C <= kWave) {
+      for (; w + 7 <= w1; w += 8) {
+        T v[8];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) v[q] = scratch[static_cast<int64_t>(w + q) * 2 * C + j];
+        for (int q = 0; q < 8; ++q) v[q] = scratch[static_cast<int64_t>(w + q) * 2 * C + j];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) s += v[q];
+        for (int q = 0; q < 8; ++q) s += v[q];
+      }
     }
     for (; w <= w1; ++w) s += scratch[static_cast<int64_t>(w) * 2 * C + j];
     locals[static_cast<int64_t>(t) * cs + j] += s;
