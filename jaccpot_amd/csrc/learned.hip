@@ -38,6 +38,7 @@ constexpr int oW1 = 0, oB1 = oW1 + kHid * kIn, oW2 = oB1 + kHid, oB2 = oW2 + kHi
 static_assert(kParams == 4931, "parameter layout");
 
 constexpr int kThreads = 256;
+constexpr int kWave = 64;
 constexpr int kSub = 32;  // pairs staged per backward sub-batch
 
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
@@ -257,7 +258,6 @@ __global__ __launch_bounds__(kThreads) void mlp_forward_mfma_kernel(int32_t K, c
                                                                     const float* __restrict__ inv_std,
                                                                     const uint16_t* __restrict__ wbf,
                                                                     float* __restrict__ Y) {
-  constexpr int kWave = 64;
   const int lane = threadIdx.x % kWave;
   const int h = lane / 32;
   const int c32 = lane % 32;
@@ -467,10 +467,14 @@ __global__ __launch_bounds__(kThreads) void select_gears_kernel(int32_t num_node
                                                                 const float* __restrict__ centers4, float tol,
                                                                 int32_t g0, int32_t g1, int32_t g2,
                                                                 int8_t* __restrict__ gears) {
-  const int t = blockIdx.x * kThreads + threadIdx.x;
-  if (t >= num_nodes) return;
+  // one wave per target node, lanes over its far pairs (far lists range from
+  // a few pairs to thousands; a thread per node left the long ones running
+  // alone); the budget is a wave maximum, exact in any order
+  const int t = blockIdx.x * (kThreads / kWave) + static_cast<int>(threadIdx.x) / kWave;
+  const int lane = static_cast<int>(threadIdx.x) % kWave;
+  if (t >= num_nodes) return;  // wave-uniform
   const int lo = far_off[t], hi = far_off[t + 1];
-  if (lo == hi) return;
+  if (lo == hi) return;  // wave-uniform
   const V4<float> ct = load4(centers4, t);
   auto weight = [&](int k) {
     const V4<float> cs4 = load4(centers4, far_src[k]);
@@ -488,8 +492,9 @@ __global__ __launch_bounds__(kThreads) void select_gears_kernel(int32_t num_node
     return extrap ? y[1] + slope * fminf(y[1] - y[0], 0.0f) : y[2];
   };
   float budget = -INFINITY;
-  for (int k = lo; k < hi; ++k) budget = fmaxf(budget, top(k) + weight(k));
-  for (int k = lo; k < hi; ++k) {
+  for (int k = lo + lane; k < hi; k += kWave) budget = fmaxf(budget, top(k) + weight(k));
+  for (int o = kWave / 2; o >= 1; o >>= 1) budget = fmaxf(budget, __shfl_xor(budget, o, kWave));
+  for (int k = lo + lane; k < hi; k += kWave) {
     const float thr = budget + tol - weight(k);
     const float* y = Y + static_cast<int64_t>(k) * kOut;
     int g = g2;
@@ -605,7 +610,7 @@ int32_t jfm_select_gears(void* stream, int32_t num_nodes, const int32_t* far_off
   if (num_nodes == 0) return JFM_OK;
   if (!far_off || !far_src || !predictions || !centers4 || !gears)
     return fail(JFM_ERR_ARG, "jfm_select_gears: NULL buffer");
-  select_gears_kernel<<<nblocks(num_nodes, kThreads), kThreads, 0, as_stream(stream)>>>(
+  select_gears_kernel<<<nblocks(num_nodes, kThreads / kWave), kThreads, 0, as_stream(stream)>>>(
       num_nodes, far_off, far_src, predictions, (const float*)centers4, tol_log10, gears3[0], gears3[1], gears3[2],
       gears);
   return check_launch("jfm_select_gears");
