@@ -286,6 +286,16 @@ int32_t jfm_l2p(void* stream, int32_t dtype, int32_t p, const void* pos4, int32_
                 int32_t num_internal, const void* locals, const void* centers4, double G, const int32_t* out_index,
                 void* acc, void* pot, int32_t accumulate);
 
+/* Combine of a deferred far field: acc[k] += -G far_acc[k] for k < 3 n and,
+ * when pot is not NULL, pot[r] += -G far_pot[r] for r < n, where far_acc /
+ * far_pot are the rows of a jfm_l2p run with G = -1 (unscaled) and
+ * accumulate == 0.  Equal bit for bit to a jfm_l2p that accumulates onto the
+ * same near-field rows; it lets L2P run before the near field is done.
+ * The far + near combine of the reference's evaluation
+ * (runtime/kernels/_evaluate.py:2106-2485, accelerations = far + near). */
+int32_t jfm_l2p_combine(void* stream, int32_t dtype, int32_t n, double G, const void* far_acc, const void* far_pot,
+                        void* acc, void* pot);
+
 /* P2P near field only (self leaf i != j plus near leaves), leaf_size <= 256.
  * flags: JFM_EVAL_POTENTIAL, JFM_EVAL_COMPENSATED.  accumulate != 0 adds into
  * the output rows instead of overwriting them.  Replaces

@@ -25,7 +25,7 @@ EVAL_FAR = 2
 EVAL_POTENTIAL = 4
 EVAL_COMPENSATED = 8
 MAX_ORDER = 8
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 _i32p = POINTER(c_int32)
 _f64p = POINTER(c_double)
@@ -88,6 +88,7 @@ _SIGNATURES = {
                               c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "jfm_l2p": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_int32, c_int32, c_int32, c_void_p, c_void_p,
                           c_double, c_void_p, c_void_p, c_void_p, c_int32]),
+    "jfm_l2p_combine": (c_int32, [c_void_p, c_int32, c_int32, c_double, c_void_p, c_void_p, c_void_p, c_void_p]),
     "jfm_p2p": (c_int32, [c_void_p, c_int32, c_void_p, c_int32, c_int32, c_int32, c_void_p, c_void_p, c_double,
                           c_double, c_void_p, c_void_p, c_void_p, c_int32, c_int32]),
     "jfm_p2p_items": (c_int32, [c_void_p, c_int32, c_void_p, c_int32, c_int32, c_int32, c_int32, c_void_p, c_void_p,

@@ -285,6 +285,19 @@ __global__ __launch_bounds__(256) void l2p_kernel(const T* __restrict__ pos4, in
   }
 }
 
+// acc[k] += -G far[k] over 3 n entries (and pot[r] += -G far_pot[r]): the
+// far field of an L2P run with G = -1 (unscaled gradient and potential rows),
+// added to near-field rows with the same expression as the L2P accumulate,
+// so the sum equals an L2P that accumulates onto those rows bit for bit.
+template <typename T>
+__global__ __launch_bounds__(256) void l2p_combine_kernel(int64_t n3, int32_t n, const T* __restrict__ far_acc,
+                                                          const T* __restrict__ far_pot, T G, T* __restrict__ acc,
+                                                          T* __restrict__ pot) {
+  const int64_t k = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (k < n3) acc[k] += -G * far_acc[k];
+  if (pot && k < n) pot[k] += -G * far_pot[k];
+}
+
 // Mixed precision: the internal nodes of the top three levels (depth <= 2,
 // at most 7 nodes) keep float64 shadow rows.  One workgroup walks their M2M
 // levels deepest first in float64, reading float32 rows for the children
@@ -571,6 +584,24 @@ int32_t jfm_l2p(void* stream, int32_t dtype, int32_t p, const void* pos4, int32_
                                                         accumulate);
   }));
   return check_launch("jfm_l2p");
+}
+
+int32_t jfm_l2p_combine(void* stream, int32_t dtype, int32_t n, double G, const void* far_acc, const void* far_pot,
+                        void* acc, void* pot) {
+  if (n < 1) return fail(JFM_ERR_ARG, "jfm_l2p_combine: bad size");
+  if (!far_acc || !acc || (pot && !far_pot)) return fail(JFM_ERR_ARG, "jfm_l2p_combine: NULL buffer");
+  hipStream_t s = as_stream(stream);
+  const int64_t n3 = 3 * static_cast<int64_t>(n);
+  const unsigned grid = static_cast<unsigned>((n3 + 255) / 256);
+  if (dtype == JFM_F32)
+    l2p_combine_kernel<float><<<grid, 256, 0, s>>>(n3, n, (const float*)far_acc, (const float*)far_pot,
+                                                   static_cast<float>(G), (float*)acc, (float*)pot);
+  else if (dtype == JFM_F64)
+    l2p_combine_kernel<double><<<grid, 256, 0, s>>>(n3, n, (const double*)far_acc, (const double*)far_pot, G,
+                                                    (double*)acc, (double*)pot);
+  else
+    return fail(JFM_ERR_ARG, "jfm_l2p_combine: bad dtype");
+  return check_launch("jfm_l2p_combine");
 }
 
 }  // extern "C"

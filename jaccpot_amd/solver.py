@@ -539,7 +539,9 @@ class FastMultipoleMethod:
         torch.cuda.Event pairs 'upward', 'downward', 'l2p' (current stream)
         and 'p2p' (its stream) recorded around each part.  ``overlap=False`` runs
         the same launches on the current stream only, P2P after the downward
-        pass."""
+        pass.  In the overlapped order L2P runs before the join into per-state
+        rows (unscaled) that jfm_l2p_combine adds to the near field after it
+        (the same expression as the accumulating L2P, so the same bits)."""
         topo = state.topology
         main = torch.cuda.current_stream(state.device)
         s = nat.current_stream()
@@ -572,15 +574,37 @@ class FastMultipoleMethod:
         mark("downward", 0)
         self._downward(state, s, code)
         mark("downward", 1)
+        mark("l2p", 0)
         if overlap:
+            # the far field is evaluated while the near field may still run:
+            # L2P with G = -1 writes unscaled rows, jfm_l2p_combine adds them
+            # after the join exactly as an accumulating L2P would
+            far_acc, far_pot = self._far_rows(state, pot is not None)
+            nat.call("jfm_l2p", s, code, state.order, state.pos4.data_ptr(), topo.num_particles, topo.leaf_size,
+                     topo.num_internal, state.locals.data_ptr(), state.centers4.data_ptr(), -1.0,
+                     state.perm.data_ptr(), far_acc.data_ptr(), far_pot.data_ptr() if pot is not None else None, 0)
             main.wait_stream(side)
+            nat.call("jfm_l2p_combine", s, code, topo.num_particles, state.G, far_acc.data_ptr(),
+                     far_pot.data_ptr() if pot is not None else None, acc.data_ptr(),
+                     pot.data_ptr() if pot is not None else None)
         else:
             near()
-        mark("l2p", 0)
-        nat.call("jfm_l2p", s, code, state.order, state.pos4.data_ptr(), topo.num_particles, topo.leaf_size,
-                 topo.num_internal, state.locals.data_ptr(), state.centers4.data_ptr(), state.G,
-                 state.perm.data_ptr(), acc.data_ptr(), pot.data_ptr() if pot is not None else None, 1)
+            nat.call("jfm_l2p", s, code, state.order, state.pos4.data_ptr(), topo.num_particles, topo.leaf_size,
+                     topo.num_internal, state.locals.data_ptr(), state.centers4.data_ptr(), state.G,
+                     state.perm.data_ptr(), acc.data_ptr(), pot.data_ptr() if pot is not None else None, 1)
         mark("l2p", 1)
+
+    @staticmethod
+    def _far_rows(state: FMMPreparedState, with_pot: bool):
+        """Per-state rows for the deferred far field of the overlapped step."""
+        n = state.topology.num_particles
+        rows = state.extra.get("far_rows")
+        if (rows is None or (with_pot and rows[1] is None) or rows[0].shape[0] != n or rows[0].dtype != state.dtype
+                or rows[0].device != state.device):
+            rows = (torch.empty((n, 3), dtype=state.dtype, device=state.device),
+                    torch.empty((n,), dtype=state.dtype, device=state.device) if with_pot else None)
+            state.extra["far_rows"] = rows
+        return rows
 
     def _side_stream(self, device) -> torch.cuda.Stream:
         st = getattr(self, "_streams", None)
