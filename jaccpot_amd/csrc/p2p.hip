@@ -143,9 +143,10 @@ __device__ __forceinline__ void fetch_positions_leaf(const T* __restrict__ pos4,
 // a ragged last leaf and slots past `total` (sent to row n) need no clamp and
 // no mask, and the LOADS rows of a thread share one offset register (the row
 // steps go into the instructions' immediate offsets).  Requires total and
-// first to be multiples of LOADS (slot ranges are whole leaves) and
-// 16 * (n + LOADS) < 2^31.
-constexpr int kBufMaxRows = (1 << 27) - 64;
+// first to be multiples of LOADS (slot ranges are whole leaves); row offsets
+// reach 16 * (n + WT) with WT <= 128 on this path, kept below 2^31.
+constexpr int kBufMaxRows = (1 << 27) - 512;
+static_assert((static_cast<int64_t>(kBufMaxRows) + 128) * 16 + 64 < (int64_t(1) << 31), "buffer offsets fit int32");
 constexpr int kBufDword3 = 0x00020000;  // untyped 32-bit data format
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t pos4_rsrc(const void* pos4, int n) {
