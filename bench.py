@@ -162,7 +162,7 @@ def main() -> int:
             run_step()
         torch.cuda.synchronize()
         events = [{k: [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-                   for k in ("upward", "downward", "l2p", "p2p")} for _ in range(args.steps)]
+                   for k in ("upward", "downward", "l2p", "combine", "p2p")} for _ in range(args.steps)]
         torch.cuda.synchronize()
         t_start = time.perf_counter()
         for k in range(args.steps):
@@ -170,7 +170,7 @@ def main() -> int:
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t_start
         ms_per_step = 1e3 * elapsed / args.steps
-        stage = {"upward": [], "downward": [], "l2p": [], "p2p": []}
+        stage = {"upward": [], "downward": [], "l2p": [], "combine": [], "p2p": []}
         for ev in events:
             for k in stage:
                 stage[k].append(ev[k][0].elapsed_time(ev[k][1]))
@@ -355,7 +355,7 @@ def measure_gradient(torch, FastMultipoleMethod, ics, args):
     mt = torch.as_tensor(m, dtype=torch.float32, device="cuda").requires_grad_(True)
 
     def step(backward: bool):
-        acc, pot = fmm.differentiable_accelerations(state, xt, mt, return_potential=True)
+        acc, pot = fmm.differentiable_accelerations_and_potential(state, xt, mt)
         loss = (mt[tgt] * pot[tgt]).sum() + 0.5 * (acc[tgt] * acc[tgt]).sum()
         if backward:
             xt.grad = None

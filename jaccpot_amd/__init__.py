@@ -6,11 +6,19 @@ evaluate_prepared_state``) on hand-written HIP kernels for gfx950, reached
 through the C ABI of ``libjfm.so`` (include/jfm.h).
 """
 
+from .config import (FarFieldConfig, FMMAdvancedConfig, GradConfig, NearFieldConfig, RuntimePolicyConfig,
+                     TreeConfig)
 from .direct import direct_sum_gravitational_acceleration
 from .solver import FastMultipoleMethod, FMMPreparedState
 from .topology import Topology, build_topology
 
 __all__ = [
+    "FarFieldConfig",
+    "FMMAdvancedConfig",
+    "GradConfig",
+    "NearFieldConfig",
+    "RuntimePolicyConfig",
+    "TreeConfig",
     "FastMultipoleMethod",
     "FMMPreparedState",
     "Topology",

@@ -122,39 +122,61 @@ class FastMultipoleMethod:
         m2l_impl: Optional[str] = None,
         adaptive_order: bool = False,
         p_gears: Optional[Sequence[int]] = None,
+        use_pallas: Optional[bool] = None,
+        reuse_topology: bool = False,
+        rebuild_every: int = 1,
+        mac_force_scale_mode: str = "prev",
+        mac_force_scale_prepass_theta: Optional[float] = None,
+        mac_force_scale_fb_inflation: float = 1.0,
+        adaptive_error_model: str = "tail_proxy",
+        adaptive_eps: Optional[float] = None,
+        dehnen_geometry_mode: str = "com",
+        mac_theta_max: float = 1.0,
         theta: float = 0.6,
         G: float = 1.0,
         softening: float = 1e-3,
         precision: Optional[str] = None,
         working_dtype: Any = None,
+        advanced: Any = None,
         compensated: bool = True,
         mixed_precision: bool = False,
         order_model: Any = None,
         gear_tolerance: float = -2.5,
+        order_judge_targets: int = 1024,
+        order_judge_factor: float = 2.0,
+        order_judge_seed: int = 0,
         num_threads: int = 0,
         tree_build: str = "device",
         split_near: bool = True,
-        reuse_topology: bool = False,
-        rebuild_every: int = 1,
         **legacy_kwargs: Any,
     ):
+        from .config import resolve_constructor_options
+
+        opts = resolve_constructor_options(
+            adaptive_order=bool(adaptive_order), use_pallas=use_pallas, mac_force_scale_mode=mac_force_scale_mode,
+            mac_force_scale_prepass_theta=mac_force_scale_prepass_theta,
+            mac_force_scale_fb_inflation=mac_force_scale_fb_inflation, adaptive_error_model=adaptive_error_model,
+            adaptive_eps=adaptive_eps, dehnen_geometry_mode=dehnen_geometry_mode, mac_theta_max=mac_theta_max,
+            advanced=advanced, legacy_kwargs=legacy_kwargs)
+        basis = opts["renamed"].get("basis", basis)
+        preset = opts["renamed"].get("preset", preset)
         preset_norm = str(getattr(preset, "value", preset)).strip().lower()
         if preset_norm not in _PRESETS:
             raise ValueError(f"unknown preset {preset!r}; expected one of {_PRESETS}")
-        if str(basis).strip().lower() != "real":
+        if str(getattr(basis, "value", basis)).strip().lower() != "real":
             raise NotImplementedError("only basis='real' (the Dehnen real basis) is implemented")
         if m2l_impl not in (None, "rot_scale"):
             raise NotImplementedError(f"m2l_impl={m2l_impl!r} is not implemented (use 'rot_scale')")
         if adaptive_order:
-            if order_model is None:
-                raise ValueError("adaptive_order=True needs a trained order_model (jaccpot_amd.learned.OrderModel)")
+            if p_gears is None and order_model is None:
+                raise ValueError("adaptive_order=True requires non-empty p_gears")
             gears = tuple(int(g) for g in (p_gears if p_gears is not None else order_model.gears))
-            if gears != tuple(order_model.gears):
+            if len(gears) != 3 or list(gears) != sorted(set(gears)) or not (1 <= gears[0] and gears[-1] <= 8):
+                raise ValueError(f"p_gears must be three increasing orders in 1..8, got {gears}")
+            if order_model is not None and gears != tuple(order_model.gears):
                 raise ValueError(f"p_gears {gears} do not match the order model's gears {order_model.gears}")
         elif p_gears is not None:
             raise ValueError("p_gears is only meaningful with adaptive_order=True")
-        if legacy_kwargs:
-            raise TypeError(f"unsupported keyword arguments: {sorted(legacy_kwargs)}")
         if not (theta > 0.0):
             raise ValueError("theta must be > 0")
         if not (softening >= 0.0):
@@ -163,6 +185,8 @@ class FastMultipoleMethod:
             raise ValueError("tree_build must be 'device' or 'host'")
         if int(rebuild_every) <= 0:
             raise ValueError("rebuild_every must be positive")
+        if int(order_judge_targets) < 0 or not (float(order_judge_factor) >= 1.0):
+            raise ValueError("order_judge_targets must be >= 0 and order_judge_factor >= 1")
         dtype = None
         if precision is not None:
             p = str(precision).strip().lower()
@@ -178,6 +202,9 @@ class FastMultipoleMethod:
             dtype = torch.float32
         self.preset = preset_norm
         self.basis = "real"
+        self.options = opts
+        self.use_pallas = use_pallas
+        self.advanced = advanced
         self.theta = float(theta)
         self.G = float(G)
         self.softening = float(softening)
@@ -185,8 +212,10 @@ class FastMultipoleMethod:
         self.compensated = bool(compensated)
         self.mixed_precision = bool(mixed_precision)
         self.adaptive_order = bool(adaptive_order)
-        self.order_model = order_model
         self.gear_tolerance = float(gear_tolerance)
+        self.order_judge_targets = int(order_judge_targets)
+        self.order_judge_factor = float(order_judge_factor)
+        self.order_judge_seed = int(order_judge_seed)
         self.num_threads = int(num_threads)
         self.tree_build = tree_build
         self.split_near = bool(split_near)
@@ -196,6 +225,22 @@ class FastMultipoleMethod:
         self._last_inputs: Optional[tuple] = None
         self.device = _require_gpu()
         self._last_state: Optional[FMMPreparedState] = None
+        if self.adaptive_order and order_model is None:
+            order_model = self._shipped_order_model(gears)
+        self.order_model = order_model
+        self.p_gears = tuple(order_model.gears) if order_model is not None else None
+
+    @staticmethod
+    def _shipped_order_model(gears):
+        """The order model trained for ``gears`` that ships with the package."""
+        from .train_order import load, weights_path
+
+        path = weights_path(gears)
+        if not path.exists():
+            raise NotImplementedError(
+                f"no trained order model ships for p_gears={gears}; train one with "
+                f"python -m jaccpot_amd.train_order --gears {','.join(map(str, gears))} and pass it as order_model")
+        return load(path)
 
     # ------------------------------------------------------------------ utils
     def _as_inputs(self, positions, masses):
@@ -225,8 +270,24 @@ class FastMultipoleMethod:
 
     # -------------------------------------------------------------- prepare
     def prepare_state(self, positions, masses, *, bounds=None, leaf_size: int = 16, max_order: int = 4,
-                      theta: Optional[float] = None) -> FMMPreparedState:
-        """Build the topology and run the upward + downward passes (solver.py:1012)."""
+                      theta: Optional[float] = None, cache_policy: str = "auto", force_scale_nodes=None,
+                      runtime_overrides_override=None, fused_device_mode: bool = False,
+                      **extra) -> FMMPreparedState:
+        """Build the topology and run the upward + downward passes (solver.py:1012).
+
+        ``cache_policy``, ``runtime_overrides_override`` and
+        ``fused_device_mode`` choose how the reference caches and lays out its
+        JAX state; they do not change the result and have no effect here.
+        With ``adaptive_order`` the learned gears are chosen here and judged
+        against the plain top-gear FMM on ``order_judge_targets`` seeded
+        targets (``jaccpot_amd.judge``); the verdict is in
+        ``state.extra["order_judge"]``."""
+        from .config import resolve_prepare_options
+
+        resolve_prepare_options(adaptive_order=self.adaptive_order, cache_policy=cache_policy,
+                                force_scale_nodes=force_scale_nodes,
+                                runtime_overrides_override=runtime_overrides_override,
+                                fused_device_mode=fused_device_mode, extra=extra)
         p = int(max_order)
         if p < 1 or p > nat.MAX_ORDER:
             raise ValueError(f"max_order must be in [1, {nat.MAX_ORDER}]")
@@ -260,6 +321,11 @@ class FastMultipoleMethod:
         if self.adaptive_order:
             self._prepare_gears(state, xd, md)
         self.refresh_prepared_state(state, xd, md)
+        if self.adaptive_order and self.order_judge_targets > 0 and state.extra.get("gears") is not None:
+            from .judge import judge_learned_orders
+
+            judge_learned_orders(self, state, xd, md, num_targets=self.order_judge_targets,
+                                 factor=self.order_judge_factor, seed=self.order_judge_seed)
         self._last_state = state
         if self.reuse_topology:
             codes = on_dev.get("sorted_codes")
@@ -500,13 +566,15 @@ class FastMultipoleMethod:
                      state.near_src.data_ptr(), state.locals.data_ptr(), state.centers4.data_ptr(),
                      state.softening ** 2, state.G, state.perm.data_ptr(), acc.data_ptr(), potp, flags)
             return
-        far = bool(flags & nat.EVAL_FAR)
-        if far:
+        # near field first, far field added onto its rows: the order of
+        # step_device, so both round a*G + (-G)*g the same way for any G
+        near = bool(flags & nat.EVAL_NEAR)
+        if near:
+            self._p2p_items(state, items, acc, pot, flags, accumulate=False)
+        if flags & nat.EVAL_FAR:
             nat.call("jfm_l2p", s, code, state.order, state.pos4.data_ptr(), topo.num_particles, topo.leaf_size,
                      topo.num_internal, state.locals.data_ptr(), state.centers4.data_ptr(), state.G,
-                     state.perm.data_ptr(), acc.data_ptr(), potp, 0)
-        if flags & nat.EVAL_NEAR:
-            self._p2p_items(state, items, acc, pot, flags, accumulate=far)
+                     state.perm.data_ptr(), acc.data_ptr(), potp, 1 if near else 0)
 
     @staticmethod
     def _p2p_items(state: FMMPreparedState, items, acc: torch.Tensor, pot: Optional[torch.Tensor], flags: int,
@@ -531,17 +599,19 @@ class FastMultipoleMethod:
                     overlap: bool = True) -> None:
         """One fixed-topology step (refresh_prepared_state + the evaluation) with the
         near field overlapped: after the gather, P2P runs on a second stream
-        while the upward and downward passes run on the current one, then L2P
-        adds the far field to the near-field rows (the two terms are the same
-        as in the sequential order, and one addition of two values does not
-        depend on their order, so the result is identical).  ``acc``/``pot``
-        are written in input order.  ``events`` (optional): dict of
-        torch.cuda.Event pairs 'upward', 'downward', 'l2p' (current stream)
-        and 'p2p' (its stream) recorded around each part.  ``overlap=False`` runs
-        the same launches on the current stream only, P2P after the downward
-        pass.  In the overlapped order L2P runs before the join into per-state
-        rows (unscaled) that jfm_l2p_combine adds to the near field after it
-        (the same expression as the accumulating L2P, so the same bits)."""
+        while the upward and downward passes run on the current one, then the
+        far field is added to the near-field rows.  ``acc``/``pot`` are written
+        in input order.  ``events`` (optional): dict of torch.cuda.Event pairs
+        'upward', 'downward', 'l2p', 'combine' (current stream) and 'p2p' (its
+        stream) recorded around each part; 'l2p' spans the L2P launch only,
+        'combine' the join with the P2P stream and the far + near add.
+        ``overlap=False`` runs the same launches on the current stream only,
+        P2P after the downward pass, then an accumulating L2P.  In the
+        overlapped order L2P runs before the join into per-state rows
+        (unscaled) that jfm_l2p_combine adds to the near-field rows after it,
+        with the accumulating L2P's expression (acc += -G * far), so both
+        orders, and _evaluate_device (also near first, then far), give the
+        same bits for any G."""
         topo = state.topology
         main = torch.cuda.current_stream(state.device)
         s = nat.current_stream()
@@ -574,35 +644,42 @@ class FastMultipoleMethod:
         mark("downward", 0)
         self._downward(state, s, code)
         mark("downward", 1)
-        mark("l2p", 0)
         if overlap:
             # the far field is evaluated while the near field may still run:
             # L2P with G = -1 writes unscaled rows, jfm_l2p_combine adds them
             # after the join exactly as an accumulating L2P would
-            far_acc, far_pot = self._far_rows(state, pot is not None)
+            far_acc, far_pot = self._far_rows(state)
+            mark("l2p", 0)
             nat.call("jfm_l2p", s, code, state.order, state.pos4.data_ptr(), topo.num_particles, topo.leaf_size,
                      topo.num_internal, state.locals.data_ptr(), state.centers4.data_ptr(), -1.0,
                      state.perm.data_ptr(), far_acc.data_ptr(), far_pot.data_ptr() if pot is not None else None, 0)
+            mark("l2p", 1)
+            mark("combine", 0)
             main.wait_stream(side)
             nat.call("jfm_l2p_combine", s, code, topo.num_particles, state.G, far_acc.data_ptr(),
                      far_pot.data_ptr() if pot is not None else None, acc.data_ptr(),
                      pot.data_ptr() if pot is not None else None)
+            mark("combine", 1)
         else:
             near()
+            mark("l2p", 0)
             nat.call("jfm_l2p", s, code, state.order, state.pos4.data_ptr(), topo.num_particles, topo.leaf_size,
                      topo.num_internal, state.locals.data_ptr(), state.centers4.data_ptr(), state.G,
                      state.perm.data_ptr(), acc.data_ptr(), pot.data_ptr() if pot is not None else None, 1)
-        mark("l2p", 1)
+            mark("l2p", 1)
+            mark("combine", 0)
+            mark("combine", 1)
 
     @staticmethod
-    def _far_rows(state: FMMPreparedState, with_pot: bool):
-        """Per-state rows for the deferred far field of the overlapped step."""
-        n = state.topology.num_particles
+    def _far_rows(state: FMMPreparedState):
+        """Per-state rows (accelerations and potentials) for the deferred far
+        field of the overlapped step.  Allocated once per state and never
+        replaced: a captured EvaluationGraph holds their addresses."""
         rows = state.extra.get("far_rows")
-        if (rows is None or (with_pot and rows[1] is None) or rows[0].shape[0] != n or rows[0].dtype != state.dtype
-                or rows[0].device != state.device):
+        if rows is None:
+            n = state.topology.num_particles
             rows = (torch.empty((n, 3), dtype=state.dtype, device=state.device),
-                    torch.empty((n,), dtype=state.dtype, device=state.device) if with_pot else None)
+                    torch.empty((n,), dtype=state.dtype, device=state.device))
             state.extra["far_rows"] = rows
         return rows
 
@@ -670,33 +747,70 @@ class FastMultipoleMethod:
         outputs are static tensors in input order."""
         return EvaluationGraph(self, state, positions, masses, return_potential)
 
-    def differentiable_accelerations(self, state: FMMPreparedState, positions, masses, *,
-                                     return_potential: bool = False):
-        """Accelerations at the state's frozen topology, differentiable w.r.t. positions and masses
-        (solver.py:1358; torch.autograd with libjfm's adjoint kernels)."""
+    def differentiable_accelerations(self, state: FMMPreparedState, positions, masses, *, target_indices=None,
+                                     jit_traversal: bool = False, grad_plan=None, grad_config=None):
+        """``(N, 3)`` accelerations (``(T, 3)`` with ``target_indices``) at the
+        state's frozen topology, differentiable w.r.t. ``positions`` and
+        ``masses`` (solver.py:1358-1368): torch.autograd with libjfm's adjoint
+        kernels, expansions recomputed in the backward pass.  All particles
+        remain sources.  ``jit_traversal`` and ``grad_config``
+        (:class:`jaccpot_amd.config.GradConfig`) select the reference's JAX
+        lanes and have no effect here; ``grad_plan`` (large-N only) is not used.
+        :meth:`differentiable_accelerations_and_potential` also returns the
+        potentials."""
+        from .config import resolve_grad_config
+
+        resolve_grad_config(grad_config, grad_plan)
+        del jit_traversal
+        acc, _ = self._differentiable(state, positions, masses, target_indices, return_potential=False)
+        return acc
+
+    def differentiable_accelerations_and_potential(self, state: FMMPreparedState, positions, masses, *,
+                                                   target_indices=None, grad_config=None):
+        """(acc, pot) of :meth:`differentiable_accelerations`, both differentiable
+        (an addition of this build: the reference's method returns accelerations only)."""
+        from .config import resolve_grad_config
+
+        resolve_grad_config(grad_config, None)
+        return self._differentiable(state, positions, masses, target_indices, return_potential=True)
+
+    def _differentiable(self, state, positions, masses, target_indices, return_potential: bool):
         from .autodiff import differentiable_accelerations
 
-        return differentiable_accelerations(self, state, positions, masses, return_potential=return_potential)
+        out = differentiable_accelerations(self, state, positions, masses, return_potential=True)
+        acc, pot = out
+        if target_indices is not None:
+            idx = torch.as_tensor(np.asarray(target_indices) if not isinstance(target_indices, torch.Tensor)
+                                  else target_indices, device=acc.device).long()
+            if idx.ndim != 1:
+                raise ValueError("target_indices must be 1-D")
+            if idx.numel() and (int(idx.min()) < 0 or int(idx.max()) >= acc.shape[0]):
+                raise ValueError("target_indices out of range")
+            acc = acc.index_select(0, idx)
+            pot = pot.index_select(0, idx)
+        return acc, (pot if return_potential else None)
 
     def compute_accelerations(self, positions, masses, *, target_indices=None, bounds=None, leaf_size: int = 16,
                               max_order: int = 4, return_potential: bool = False, theta: Optional[float] = None,
                               reuse_prepared_state: bool = False, max_acc_derivative_order: int = 0):
         """prepare_state + evaluate_prepared_state (solver.py:917).
 
-        ``reuse_prepared_state``: evaluate the last prepared state again when the
-        position and mass objects and the preparation arguments are the same
-        objects / values as in the call that prepared it (an identity check, like
-        the reference's: an array mutated in place is not noticed)."""
+        ``reuse_prepared_state``: evaluate the state this method prepared last
+        again when the position and mass objects and the preparation arguments
+        are the same objects / values as in that call (an identity check, like
+        the reference's: an array mutated in place is not noticed).  The state
+        is cached with its key (fmm_derivatives.py:141-185), so direct
+        ``prepare_state`` calls in between do not change what is reused."""
         args = (int(leaf_size), int(max_order), theta)
         last = self._last_inputs
-        if (reuse_prepared_state and bounds is None and self._last_state is not None and last is not None
+        if (reuse_prepared_state and bounds is None and last is not None
                 and last[0]() is positions and last[1]() is masses and last[2] == args):
-            state = self._last_state
+            state = last[3]
         else:
             state = self.prepare_state(positions, masses, bounds=bounds, leaf_size=leaf_size, max_order=max_order,
                                        theta=theta)
             try:
-                self._last_inputs = (weakref.ref(positions), weakref.ref(masses), args)
+                self._last_inputs = (weakref.ref(positions), weakref.ref(masses), args, state)
             except TypeError:  # objects without weak references (lists) are never reused
                 self._last_inputs = None
         return self.evaluate_prepared_state(state, target_indices=target_indices, return_potential=return_potential,
@@ -728,6 +842,9 @@ class EvaluationGraph:
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
             step()
+        # the replay reads and writes these buffers by address: keep them alive
+        self.state = state
+        self.far_rows = state.extra.get("far_rows")
 
     def replay(self):
         self.graph.replay()

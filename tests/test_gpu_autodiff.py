@@ -131,14 +131,14 @@ def test_autograd_function_and_fp32(require_gpu):
     st32 = fmm32.prepare_state(x.astype(np.float32), m.astype(np.float32), leaf_size=16, max_order=4)
     xt = torch.as_tensor(x, dtype=torch.float32, device="cuda").requires_grad_(True)
     mt = torch.as_tensor(m, dtype=torch.float32, device="cuda").requires_grad_(True)
-    acc, pot = fmm32.differentiable_accelerations(st32, xt, mt, return_potential=True)
+    acc, pot = fmm32.differentiable_accelerations_and_potential(st32, xt, mt)
     loss = (mt * pot).sum() + 0.5 * (acc * acc).sum()
     loss.backward()
     fmm64 = FastMultipoleMethod(precision="fp64", theta=0.6, softening=1e-2)
     st64 = fmm64.prepare_state(x, m, leaf_size=16, max_order=4)
     x64 = torch.as_tensor(x, device="cuda").requires_grad_(True)
     m64 = torch.as_tensor(m, device="cuda").requires_grad_(True)
-    a64, p64 = fmm64.differentiable_accelerations(st64, x64, m64, return_potential=True)
+    a64, p64 = fmm64.differentiable_accelerations_and_potential(st64, x64, m64)
     l64 = (m64 * p64).sum() + 0.5 * (a64 * a64).sum()
     l64.backward()
     ex = (torch.linalg.norm(xt.grad.double() - x64.grad) / torch.linalg.norm(x64.grad)).item()

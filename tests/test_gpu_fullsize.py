@@ -1,0 +1,218 @@
+"""GPU parity at the full sizes of the plain, learned-order and differentiable
+workloads, against the C++ restatement of the FMM on the same topology
+(float64, same per-pair orders) and against the float64 direct sum on 4096
+seeded targets:
+
+* Plummer N=1,048,576 (seed 1), float32, p 4, leaf 32, theta 0.6;
+* composite galaxy N=2,097,152 (seed 3; Hernquist halo 80% + exponential disc
+  20%), float32 with float64 top levels, per-pair gears (2, 4, 6) from the
+  shipped order model, judged against the plain order-6 FMM;
+* 64 Gaussian blobs N=1,048,576 (seed 4), float32, p 4, leaf 32: gradients of
+  sum_t m_t phi_t + 0.5 |a_t|^2 over 1024 targets, float32 adjoint against the
+  float64 adjoint, float64 adjoint against central differences (h = 1e-3, 64
+  seeded unit directions over all positions and masses), and at an N=4096
+  subsample against the direct-sum gradient.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import ics
+from oracle.direct import direct_sum, direct_sum_vjp
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+ERR_TARGETS = 4096
+ERR_SEED = 12345
+
+
+def _cfmm():
+    from oracle import cfmm
+
+    if not cfmm.LIBRARY.exists():
+        cfmm.build()
+    return cfmm
+
+
+def _direct_errors(acc, x, m, eps):
+    """Per-target relative force errors on the 4096 seeded targets."""
+    from jaccpot_amd import direct_sum_gravitational_acceleration
+
+    tgt = np.sort(np.random.default_rng(ERR_SEED).choice(x.shape[0], ERR_TARGETS, replace=False))
+    ref = direct_sum_gravitational_acceleration(x.astype(np.float64), m.astype(np.float64), softening=eps,
+                                                target_indices=tgt).cpu().numpy()
+    got = np.asarray(acc, np.float64)[tgt]
+    return np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)
+
+
+@pytest.mark.timeout(600)
+def test_plummer_1m_p4_matches_cpu_restatement(require_gpu):
+    """Plummer N=1,048,576, float32, p 4, leaf 32: relative L2 < 1e-5 against the
+    float64 restatement on the same topology; force error on 4096 targets."""
+    from jaccpot_amd import FastMultipoleMethod
+
+    print("[gpu] plummer 1M p4 leaf32 fp32 vs C++ restatement", flush=True)
+    x, m = ics.plummer(1_048_576, 1)
+    x32, m32 = x.astype(np.float32), m.astype(np.float32)
+    fmm = FastMultipoleMethod(precision="fp32", theta=0.6, softening=1e-3)
+    state = fmm.prepare_state(x32, m32, leaf_size=32, max_order=4)
+    acc, pot = fmm.evaluate_prepared_state(state, return_potential=True)
+    torch.cuda.synchronize()
+    ref, rpot = _cfmm().evaluate(state.topology, x32.astype(np.float64), m32.astype(np.float64), p=4, eps=1e-3,
+                                 return_potential=True)
+    ra = np.linalg.norm(acc - ref) / np.linalg.norm(ref)
+    rp = np.linalg.norm(pot - rpot) / np.linalg.norm(rpot)
+    e = _direct_errors(acc, x32, m32, 1e-3)
+    print(f"[gpu]   rel-L2 vs restatement: acc {ra:.3e} pot {rp:.3e}; vs FP64 direct sum on {e.size} targets: "
+          f"median {np.median(e):.3e} p99 {np.percentile(e, 99):.3e}", flush=True)
+    assert ra < 1e-5 and rp < 1e-5
+    assert np.median(e) < 1e-3
+
+
+@pytest.mark.timeout(900)
+def test_galaxy_2m_learned_gears_246(require_gpu):
+    """Composite galaxy N=2,097,152, gears (2, 4, 6) from the shipped model
+    (leaf 16, theta 0.7, gear tolerance -2.5, float64 top levels): the judge
+    keeps the learned gears; the accelerations match the float64 restatement
+    with the same per-pair orders on the same topology to 1e-5; the learned
+    median force error stays within the judge's factor of the plain order-6
+    FMM's on 4096 targets, at fewer M2L flops."""
+    from jaccpot_amd import FastMultipoleMethod
+
+    print("[gpu] galaxy 2M learned gears (2,4,6) vs C++ restatement and FP64 direct sum", flush=True)
+    x, m = ics.galaxy(2_097_152, 3)
+    x32, m32 = x.astype(np.float32), m.astype(np.float32)
+    leaf, theta, tol = 16, 0.7, -2.5
+    fmm = FastMultipoleMethod(precision="fp32", theta=theta, softening=1e-3, mixed_precision=True,
+                              adaptive_order=True, p_gears=(2, 4, 6), gear_tolerance=tol)
+    state = fmm.prepare_state(x32, m32, leaf_size=leaf, max_order=6)
+    verdict = state.extra["order_judge"]
+    print(f"[gpu]   judge: {verdict}", flush=True)
+    assert verdict["accepted"]
+    acc = fmm.evaluate_prepared_state(state)
+    torch.cuda.synchronize()
+    orders = state.extra["gear_of_pair"]
+    counts = {int(g): int(np.sum(orders == g)) for g in (2, 4, 6)}
+    print(f"[gpu]   far pairs {orders.size}, gears {counts}; running the C++ restatement", flush=True)
+    assert sum(counts.values()) == orders.size and counts[2] + counts[4] > 0
+    ref = _cfmm().evaluate(state.topology, x32.astype(np.float64), m32.astype(np.float64), p=6, eps=1e-3,
+                           orders=orders)
+    rel = np.linalg.norm(acc - ref) / np.linalg.norm(ref)
+    e_l = _direct_errors(acc, x32, m32, 1e-3)
+    plain = FastMultipoleMethod(precision="fp32", theta=theta, softening=1e-3, mixed_precision=True)
+    sp = plain.prepare_state(x32, m32, leaf_size=leaf, max_order=6)
+    e_p = _direct_errors(plain.evaluate_prepared_state(sp), x32, m32, 1e-3)
+    sched_flops = state.extra["gears"].m2l_flops
+    from jaccpot_amd.perf import m2l_flops
+
+    frac = sched_flops / (m2l_flops(6) * orders.size)
+    print(f"[gpu]   rel-L2 vs restatement {rel:.3e}; FP64 direct sum, learned median {np.median(e_l):.3e} "
+          f"p99 {np.percentile(e_l, 99):.3e}; plain p=6 median {np.median(e_p):.3e} "
+          f"p99 {np.percentile(e_p, 99):.3e}; learned M2L flops {frac:.3f} of plain", flush=True)
+    assert rel < 1e-5
+    assert np.median(e_l) <= fmm.order_judge_factor * np.median(e_p) + 2e-6
+    assert frac < 1.0
+
+
+def _fd_loss(fmm, state, x, m, tgt):
+    """sum_t m_t phi_t + 0.5 |a_t|^2 over the targets, evaluated at (x, m) on the frozen topology."""
+    xd = torch.as_tensor(x, device="cuda").to(state.dtype)
+    md = torch.as_tensor(m, device="cuda").to(state.dtype)
+    fmm.refresh_prepared_state(state, xd, md)
+    n = x.shape[0]
+    acc = torch.empty((n, 3), dtype=state.dtype, device="cuda")
+    pot = torch.empty((n,), dtype=state.dtype, device="cuda")
+    fmm._evaluate_device(state, acc, pot)
+    t = torch.as_tensor(tgt, device="cuda")
+    return float((md[t] * pot[t]).sum() + 0.5 * (acc[t] * acc[t]).sum())
+
+
+def _grad(fmm, state, x, m, tgt):
+    """Gradient of the C5 loss through torch autograd over the adjoint kernels."""
+    xt = torch.as_tensor(x, device="cuda").to(state.dtype).requires_grad_(True)
+    mt = torch.as_tensor(m, device="cuda").to(state.dtype).requires_grad_(True)
+    acc, pot = fmm.differentiable_accelerations_and_potential(state, xt, mt)
+    t = torch.as_tensor(tgt, device="cuda")
+    loss = (mt[t] * pot[t]).sum() + 0.5 * (acc[t] * acc[t]).sum()
+    loss.backward()
+    return xt.grad.double().cpu().numpy(), mt.grad.double().cpu().numpy(), float(loss)
+
+
+@pytest.mark.timeout(900)
+def test_blobs_1m_gradient_protocol(require_gpu):
+    """Blobs N=1,048,576, p 4, leaf 32, theta 0.6, softening 1e-3; loss over
+    1024 seeded targets.  float32 adjoint vs float64 adjoint; float64 adjoint
+    vs central differences with h = 1e-3 along 64 seeded unit directions."""
+    from jaccpot_amd import FastMultipoleMethod
+
+    n = 1_048_576
+    print("[gpu] blobs 1M gradient: fp32 vs fp64 adjoint, central differences h=1e-3 x 64", flush=True)
+    x, m = ics.blobs(n, 4)
+    tgt = np.sort(np.random.default_rng(ERR_SEED).choice(n, 1024, replace=False))
+    x32, m32 = x.astype(np.float32), m.astype(np.float32)
+    f32 = FastMultipoleMethod(precision="fp32", theta=0.6, softening=1e-3)
+    s32 = f32.prepare_state(x32, m32, leaf_size=32, max_order=4)
+    gx32, gm32, _ = _grad(f32, s32, x32, m32, tgt)
+    x64, m64 = x32.astype(np.float64), m32.astype(np.float64)
+    f64 = FastMultipoleMethod(precision="fp64", theta=0.6, softening=1e-3)
+    s64 = f64.prepare_state(x64, m64, leaf_size=32, max_order=4)
+    assert np.array_equal(s64.topology.far_src, s32.topology.far_src)  # same topology for both precisions
+    gx64, gm64, _ = _grad(f64, s64, x64, m64, tgt)
+    ex = np.linalg.norm(gx32 - gx64) / np.linalg.norm(gx64)
+    em = np.linalg.norm(gm32 - gm64) / np.linalg.norm(gm64)
+    print(f"[gpu]   fp32 vs fp64 adjoint: positions {ex:.3e} masses {em:.3e}", flush=True)
+    assert ex < 1e-4 and em < 1e-4
+
+    rng = np.random.default_rng(5)
+    h = 1e-3
+    fds, ans = [], []
+    for k in range(64):
+        vx = rng.normal(size=x64.shape)
+        vm = rng.normal(size=m64.shape) * np.abs(m64).mean()
+        norm = np.sqrt((vx * vx).sum() + (vm * vm).sum())
+        vx, vm = vx / norm, vm / norm
+        fd = (_fd_loss(f64, s64, x64 + h * vx, m64 + h * vm, tgt)
+              - _fd_loss(f64, s64, x64 - h * vx, m64 - h * vm, tgt)) / (2 * h)
+        an = float((gx64 * vx).sum() + (gm64 * vm).sum())
+        fds.append(fd)
+        ans.append(an)
+        if k % 16 == 0:
+            print(f"[gpu]   dir {k}: fd {fd:.10e} adjoint {an:.10e}", flush=True)
+    fds, ans = np.asarray(fds), np.asarray(ans)
+    rms = float(np.sqrt(np.mean(ans ** 2)))
+    rel = np.abs(fds - ans) / np.maximum(np.abs(ans), rms)
+    print(f"[gpu]   central differences over 64 directions: max rel {rel.max():.3e} median {np.median(rel):.3e}",
+          flush=True)
+    assert rel.max() < 1e-6
+
+
+@pytest.mark.timeout(600)
+def test_blobs_subsample_gradient_matches_direct_sum(require_gpu):
+    """An N=4096 subsample of the blobs at the same settings (float32, p 4,
+    leaf 32, theta 0.6, softening 1e-3): the FMM gradient of the loss over
+    1024 targets against the float64 direct-sum gradient."""
+    from jaccpot_amd import FastMultipoleMethod
+
+    xb, mb = ics.blobs(1_048_576, 4)
+    sub = np.sort(np.random.default_rng(6).choice(xb.shape[0], 4096, replace=False))
+    x, m = xb[sub].astype(np.float32), mb[sub].astype(np.float32)
+    tgt = np.sort(np.random.default_rng(ERR_SEED).choice(4096, 1024, replace=False))
+    fmm = FastMultipoleMethod(precision="fp32", theta=0.6, softening=1e-3)
+    st = fmm.prepare_state(x, m, leaf_size=32, max_order=4)
+    gx, gm, _ = _grad(fmm, st, x, m, tgt)
+    x64, m64 = x.astype(np.float64), m.astype(np.float64)
+    ad, pd = direct_sum(x64, m64, eps=1e-3, return_potential=True)
+    wd = np.zeros((4096, 3))
+    ud = np.zeros(4096)
+    wd[tgt] = ad[tgt]
+    ud[tgt] = m64[tgt]
+    rx, rm = direct_sum_vjp(x64, m64, wd, ud, eps=1e-3)
+    rm[tgt] += pd[tgt]  # d/dm_t of m_t phi_t
+    ex = np.linalg.norm(gx - rx) / np.linalg.norm(rx)
+    em = np.linalg.norm(gm - rm) / np.linalg.norm(rm)
+    print(f"[gpu] blobs 4096 subsample, fp32 p4: gradient vs direct-sum gradient positions {ex:.3e} "
+          f"masses {em:.3e}", flush=True)
+    assert ex < 1e-2 and em < 1e-2

@@ -184,7 +184,8 @@ def test_trained_network_gains_over_fixed_order(require_gpu):
     best = None
     for tol in (-3.0, -2.5, -2.25, -2.0, -1.75, -1.5):
         learned = FastMultipoleMethod(precision="fp32", theta=0.6, softening=1e-3, adaptive_order=True,
-                                      p_gears=(2, 4, 6), order_model=model, gear_tolerance=tol)
+                                      p_gears=(2, 4, 6), order_model=model, gear_tolerance=tol,
+                                      order_judge_targets=0)
         stl = learned.prepare_state(x32, m32, leaf_size=32, max_order=6)
         e = err(learned.evaluate_prepared_state(stl))
         sched = stl.extra["gears"]
@@ -235,7 +236,7 @@ def test_shipped_gear368_model_cuts_order8_work(require_gpu):
     e8 = err(plain.evaluate_prepared_state(st))
     K = st.topology.num_far_pairs
     learned = FastMultipoleMethod(precision="fp32", theta=0.85, softening=1e-3, adaptive_order=True,
-                                  p_gears=(3, 6, 8), order_model=model, gear_tolerance=-1.5)
+                                  p_gears=(3, 6, 8), order_model=model, gear_tolerance=-1.5, order_judge_targets=0)
     stl = learned.prepare_state(x32, m32, leaf_size=8, max_order=8)
     el = err(learned.evaluate_prepared_state(stl))
     sched = stl.extra["gears"]

@@ -261,7 +261,7 @@ def test_graph_replay_matches_eager(learned, require_gpu):
         from jaccpot_amd.train_order import load
 
         kw = dict(mixed_precision=True, adaptive_order=True, p_gears=(2, 4, 6), order_model=load(),
-                  gear_tolerance=-2.0)
+                  gear_tolerance=-2.0, order_judge_targets=0)
     fmm = FastMultipoleMethod(precision="fp32", theta=0.6, softening=1e-3, **kw)
     state = fmm.prepare_state(x, m, leaf_size=8, max_order=6)
     xd = torch.as_tensor(x, device="cuda")
@@ -421,11 +421,11 @@ def test_reuse_prepared_state_by_identity(require_gpu):
     x, m = ics.uniform(3000, 4)
     fmm = FastMultipoleMethod(precision="fp64", theta=0.6, softening=1e-3)
     a = fmm.compute_accelerations(x, m, leaf_size=16, max_order=4)
-    st = fmm._last_state
+    st = fmm._last_inputs[3]
     b = fmm.compute_accelerations(x, m, leaf_size=16, max_order=4, reuse_prepared_state=True)
-    assert fmm._last_state is st and np.array_equal(a, b)
+    assert fmm._last_inputs[3] is st and np.array_equal(a, b)
     c = fmm.compute_accelerations(x.copy(), m, leaf_size=16, max_order=4, reuse_prepared_state=True)
-    assert fmm._last_state is not st and np.array_equal(a, c)
+    assert fmm._last_inputs[3] is not st and np.array_equal(a, c)
 
 
 @pytest.mark.parametrize("learned", [False, True], ids=["plain", "learned_mixed"])
@@ -442,7 +442,7 @@ def test_saved_state_evaluates_identically(learned, tmp_path, require_gpu):
         from jaccpot_amd.train_order import load, weights_path
 
         kw = dict(mixed_precision=True, adaptive_order=True, p_gears=(3, 6, 8),
-                  order_model=load(weights_path((3, 6, 8))), gear_tolerance=-1.5)
+                  order_model=load(weights_path((3, 6, 8))), gear_tolerance=-1.5, order_judge_targets=0)
         leaf, p = 8, 8
     fmm = FastMultipoleMethod(precision="fp32", theta=0.8, softening=1e-3, **kw)
     st = fmm.prepare_state(x, m, leaf_size=leaf, max_order=p)
@@ -460,13 +460,16 @@ def test_saved_state_evaluates_identically(learned, tmp_path, require_gpu):
     assert np.array_equal(fmm.evaluate_prepared_state(st), other.evaluate_prepared_state(st2))
 
 
+@pytest.mark.parametrize("G", [1.0, 4.3e-3], ids=["G1", "G4.3e-3"])
 @pytest.mark.parametrize("learned", [False, True], ids=["plain", "learned_mixed"])
-def test_overlapped_step_matches_sequential(learned, require_gpu):
+def test_overlapped_step_matches_sequential(learned, G, require_gpu):
     """step_device (P2P on a second stream during the upward/downward passes,
-    L2P added afterwards) equals refresh_prepared_state + evaluation bit for bit."""
+    L2P added afterwards) equals its one-stream order and refresh_prepared_state
+    + evaluation bit for bit, also for G != 1 (where near + far must be rounded
+    in the same order on every path)."""
     from jaccpot_amd import FastMultipoleMethod
 
-    print(f"[gpu] overlapped step learned={learned}", flush=True)
+    print(f"[gpu] overlapped step learned={learned} G={G}", flush=True)
     x, m = ics.plummer(50000, 23)
     x, m = x.astype(np.float32), m.astype(np.float32)
     kw, leaf, p = {}, 16, 4
@@ -474,9 +477,9 @@ def test_overlapped_step_matches_sequential(learned, require_gpu):
         from jaccpot_amd.train_order import load, weights_path
 
         kw = dict(mixed_precision=True, adaptive_order=True, p_gears=(3, 6, 8),
-                  order_model=load(weights_path((3, 6, 8))), gear_tolerance=-1.5)
+                  order_model=load(weights_path((3, 6, 8))), gear_tolerance=-1.5, order_judge_targets=0)
         leaf, p = 8, 8
-    fmm = FastMultipoleMethod(precision="fp32", theta=0.8, softening=1e-3, **kw)
+    fmm = FastMultipoleMethod(precision="fp32", theta=0.8, softening=1e-3, G=G, **kw)
     st = fmm.prepare_state(x, m, leaf_size=leaf, max_order=p)
     xd = torch.as_tensor(x * np.float32(1.0005), device="cuda")
     md = torch.as_tensor(m, device="cuda")
@@ -484,9 +487,53 @@ def test_overlapped_step_matches_sequential(learned, require_gpu):
     acc_o = torch.empty((n, 3), dtype=torch.float32, device="cuda")
     pot_o = torch.empty((n,), dtype=torch.float32, device="cuda")
     fmm.step_device(st, xd, md, acc_o, pot_o)
+    acc_q = torch.empty_like(acc_o)
+    pot_q = torch.empty_like(pot_o)
+    fmm.step_device(st, xd, md, acc_q, pot_q, overlap=False)
     acc_s = torch.empty_like(acc_o)
     pot_s = torch.empty_like(pot_o)
     fmm.refresh_prepared_state(st, xd, md)
     fmm._evaluate_device(st, acc_s, pot_s)
     torch.cuda.synchronize()
+    assert torch.equal(acc_o, acc_q) and torch.equal(pot_o, pot_q)
     assert torch.equal(acc_o, acc_s) and torch.equal(pot_o, pot_s)
+
+
+def test_graph_replay_after_potential_step(require_gpu):
+    """A graph captured without potentials still replays correctly after an
+    eager step that asked for them (the deferred far-field rows it captured
+    must stay the state's rows)."""
+    from jaccpot_amd import FastMultipoleMethod
+
+    print("[gpu] graph replay after a potential step", flush=True)
+    x, m = ics.plummer(30000, 6)
+    x, m = x.astype(np.float32), m.astype(np.float32)
+    fmm = FastMultipoleMethod(precision="fp32", theta=0.7, softening=1e-3)
+    st = fmm.prepare_state(x, m, leaf_size=16, max_order=4)
+    xd = torch.as_tensor(x, device="cuda")
+    md = torch.as_tensor(m, device="cuda")
+    g = fmm.capture_evaluation(st, xd.clone(), md.clone(), return_potential=False)
+    n = x.shape[0]
+    acc_p = torch.empty((n, 3), dtype=torch.float32, device="cuda")
+    pot_p = torch.empty((n,), dtype=torch.float32, device="cuda")
+    fmm.step_device(st, xd, md, acc_p, pot_p)
+    junk = [torch.full((n, 4), 7.0, device="cuda") for _ in range(4)]  # reuse freed blocks, if any
+    for k in range(2):
+        acc_g = g.replay().clone()
+        torch.cuda.synchronize()
+        assert torch.equal(acc_g, acc_p), k
+    del junk
+
+
+def test_reuse_prepared_state_ignores_interleaved_prepare(require_gpu):
+    """compute_accelerations(reuse_prepared_state=True) reuses the state it
+    prepared for the same objects, not whatever prepare_state built last."""
+    from jaccpot_amd import FastMultipoleMethod
+
+    x, m = ics.uniform(3000, 7)
+    y, m2 = ics.plummer(2500, 8)
+    fmm = FastMultipoleMethod(precision="fp64", theta=0.6, softening=1e-3)
+    a = fmm.compute_accelerations(x, m, leaf_size=16, max_order=4)
+    fmm.prepare_state(y, m2, leaf_size=16, max_order=4)
+    b = fmm.compute_accelerations(x, m, leaf_size=16, max_order=4, reuse_prepared_state=True)
+    assert b.shape == a.shape and np.array_equal(a, b)
