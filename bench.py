@@ -60,26 +60,41 @@ LEARNED = {
 # order-model training configuration, at its own fastest setting.
 LEARNED_246 = {"leaf": 8, "theta": 0.7, "gears": (2, 4, 6), "gear_tolerance": -1.5}
 
+# The plain fixed-order float32 FMM (no gears, no float64 levels) at the order,
+# leaf size and opening angle with the lowest step time at median force error
+# <= 1e-4 on the same targets (a sweep over p 4..8, leaf 8..64, theta 0.5..0.9):
+# the like-for-like baseline of the learned+mixed configuration.
+PLAIN_TUNED = {
+    "plummer4m_p6": {"p": 8, "leaf": 8, "theta": 0.85},
+}
+
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(x, m, p: int, leaf: int, theta: float, eps: float):
+def cpu_baseline(x, m, p: int, leaf: int, theta: float, eps: float, runs: int = 5):
     """The plain fixed-order FMM timed on the host: oracle/cfmm.cpp (the C++/OpenMP
     restatement of the reference's pipeline), float32 arithmetic, every OpenMP
-    thread, one evaluation (gather, P2M, M2M, M2L, L2L, L2P + P2P) on the native
-    host topology, which is built first and reported separately (as on the GPU)."""
+    thread, evaluations (gather, P2M, M2M, M2L, L2L, L2P + P2P) on the native
+    host topology, which is built first and reported separately (as on the GPU).
+    One warm-up evaluation, then the median of ``runs`` timed ones."""
     from jaccpot_amd.topology import build_topology
     from oracle import cfmm
 
     topo = build_topology(x, m, leaf_size=leaf, theta=theta)
     threads = int(cfmm.load().cfmm_max_threads())
-    stages = {}
-    t0 = time.perf_counter()
-    cfmm.evaluate(topo, x, m, p=p, eps=eps, precision="fp32", timings=stages)
-    dt = time.perf_counter() - t0
-    return x.shape[0] / dt, dt, threads, stages, topo.build_seconds
+    cfmm.evaluate(topo, x, m, p=p, eps=eps, precision="fp32")  # warm-up
+    times, stage_runs = [], []
+    for _ in range(runs):
+        stages = {}
+        t0 = time.perf_counter()
+        cfmm.evaluate(topo, x, m, p=p, eps=eps, precision="fp32", timings=stages)
+        times.append(time.perf_counter() - t0)
+        stage_runs.append(stages)
+    k = int(np.argsort(times)[len(times) // 2])
+    dt = float(times[k])
+    return x.shape[0] / dt, dt, threads, stage_runs[k], topo.build_seconds, times
 
 
 def main() -> int:
@@ -100,6 +115,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=0,
                     help="particles of the CPU baseline run (0: the full workload)")
+    ap.add_argument("--cpu-runs", type=int, default=5, help="timed CPU baseline runs (median; one warm-up first)")
     ap.add_argument("--error-targets", type=int, default=4096)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="run one torch copy of this many MiB first (HBM counter calibration)")
@@ -227,15 +243,22 @@ def main() -> int:
                 "p2p_interactions": int(topo.p2p_interactions())}
         if sched is not None:
             info["gear_counts"] = {str(k): v for k, v in sched.counts.items()}
+        if "order_judge" in state.extra:
+            info["order_judge"] = state.extra["order_judge"]
         del state, xd, md, acc
         torch.cuda.empty_cache()
         return info
 
     ref_cache = {}
-    plain = None
+    plain = plain_tuned = None
     if args.mode != "learned-only":
         plain = measure(FastMultipoleMethod(precision=precision, theta=theta, softening=eps, compensated=True), leaf,
                         p, "plain")
+        tuned = PLAIN_TUNED.get(args.workload)
+        if tuned is not None and args.mode == "learned":
+            plain_tuned = measure(FastMultipoleMethod(precision=precision, theta=tuned["theta"], softening=eps,
+                                                      compensated=True), tuned["leaf"], tuned["p"], "plain_tuned")
+            plain_tuned["config"] = dict(tuned)
     learned = learned246 = None
     if learned_cfg is not None:
         from jaccpot_amd.train_order import load, weights_path
@@ -264,16 +287,23 @@ def main() -> int:
     p2p_flops = 20.0 * interactions
     p2p_s = head["p2p_alone_ms"] * 1e-3
     achieved = p2p_flops / p2p_s / 1e12
-    traffic = None
+    traffic, traffic_note = None, None
     tag = "learned" if learned is not None else "plain"
     tfile = os.path.join(ROOT, "profiles", f"p2p_{args.workload}_{tag}_traffic.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            tj = json.load(f)
+        if tj.get("p2p_source_sha256") == p2p_source_hash():
+            traffic = tj.get("hbm_bytes_per_launch")
+            traffic_note = (f"{os.path.relpath(tfile, ROOT)}: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this "
+                            f"P2P source (sha256 {tj['p2p_source_sha256'][:12]}), scaled by a plain copy kernel")
+        else:
+            traffic_note = f"{os.path.relpath(tfile, ROOT)} was measured on another P2P source: not reported"
     roofline = {"bound": "valu", "kernel": "p2p_kernel", "achieved": round(achieved, 3),
                 "peak": PEAK_FP32_TFLOPS if precision == "fp32" else 78.6, "unit": "TFLOP/s",
                 "frac": round(achieved / (PEAK_FP32_TFLOPS if precision == "fp32" else 78.6), 4),
-                "traffic": traffic, "interactions_per_launch": int(interactions), "flops_per_interaction": 20,
+                "traffic": traffic, "traffic_source": traffic_note,
+                "interactions_per_launch": int(interactions), "flops_per_interaction": 20,
                 "avg_launch_ms": round(head["p2p_alone_ms"], 4),
                 "measured": "P2P (p2p_kernel + split-leaf reduction) launched alone after the timed steps, HIP "
                             "events on its stream; inside the overlapped step it runs beside the tree passes"}
@@ -286,14 +316,16 @@ def main() -> int:
     if not args.no_cpu_baseline:
         cn = N if args.cpu_sample <= 0 else min(args.cpu_sample, N)
         xc, mc = (x, m) if cn == N else getattr(ics, gen)(cn, seed)
-        rate, dt, threads, stages, topo_s = cpu_baseline(xc, mc, p, leaf, theta, eps)
+        rate, dt, threads, stages, topo_s, times = cpu_baseline(xc, mc, p, leaf, theta, eps, args.cpu_runs)
         cpu = {"value": rate, "unit": "particles/s", "cores": threads, "kind": "port",
                "sample": f"C++/OpenMP restatement of the plain fixed-order FMM (oracle/cfmm.cpp), float32, "
-                         f"{gen} N={cn} p={p} leaf={leaf} theta={theta} softening={eps}: one evaluation "
-                         f"(gather, P2M, M2M, M2L, L2L, L2P+P2P) in {dt:.1f}s on {threads} threads; "
-                         f"host topology build {topo_s:.1f}s not included",
+                         f"{gen} N={cn} p={p} leaf={leaf} theta={theta} softening={eps}: evaluation "
+                         f"(gather, P2M, M2M, M2L, L2L, L2P+P2P), median of {len(times)} after one warm-up, "
+                         f"{dt:.2f}s on {threads} threads; host topology build {topo_s:.1f}s not included",
+               "runs_s": [round(t, 3) for t in times],
                "stage_s": {k: round(v, 3) for k, v in stages.items()},
-               "cpu": platform_cpu()}
+               "cpu": platform_cpu(), "nproc": os.cpu_count(),
+               "affinity_cpus": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
         log(f"cpu baseline {rate:.3e} particles/s on N={cn} ({dt:.1f}s, {threads} threads) "
             f"stages {json.dumps({k: round(v, 2) for k, v in stages.items()})}")
 
@@ -329,10 +361,15 @@ def main() -> int:
         "force_error": head["force_error"],
         "stage_ms": stage_ms,
         "plain": plain,
+        "plain_tuned": plain_tuned,
         "learned_mixed": learned,
         "learned_mixed_gears246": dict(learned246, config=LEARNED_246) if learned246 else None,
         "differentiable": grad,
         "speedup_vs_plain": (learned["particles_per_s"] / plain["particles_per_s"]) if learned and plain else None,
+        "speedup_vs_plain_tuned": (learned["particles_per_s"] / plain_tuned["particles_per_s"])
+        if learned and plain_tuned else None,
+        "p99_force_error": {k: (v["force_error"] or {}).get("p99") for k, v in
+                            (("plain", plain), ("plain_tuned", plain_tuned), ("learned_mixed", learned)) if v},
     }
     print(json.dumps(out), flush=True)
     return 0
@@ -383,15 +420,32 @@ def measure_gradient(torch, FastMultipoleMethod, ics, args):
             "particles_per_s_forward_backward": n / both}
 
 
+def p2p_source_hash() -> str:
+    """sha256 of the P2P kernel source (ties a committed traffic figure to the kernel it measured)."""
+    import hashlib
+
+    with open(os.path.join(ROOT, "jaccpot_amd", "csrc", "p2p.hip"), "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
 def platform_cpu() -> str:
+    """CPU model string with family/model/stepping and socket count from /proc/cpuinfo."""
+    info, sockets = {}, set()
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if ":" not in line:
+                    continue
+                k, v = (t.strip() for t in line.split(":", 1))
+                if k == "physical id":
+                    sockets.add(v)
+                elif k in ("model name", "cpu family", "model", "stepping", "cpu MHz") and k not in info:
+                    info[k] = v
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    name = info.get("model name", "unknown")
+    return (f"{name} (family {info.get('cpu family', '?')} model {info.get('model', '?')} stepping "
+            f"{info.get('stepping', '?')}, {len(sockets) or '?'} socket(s), {info.get('cpu MHz', '?')} MHz)")
 
 
 if __name__ == "__main__":
