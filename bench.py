@@ -117,6 +117,8 @@ def main() -> int:
                     help="particles of the CPU baseline run (0: the full workload)")
     ap.add_argument("--cpu-runs", type=int, default=5, help="timed CPU baseline runs (median; one warm-up first)")
     ap.add_argument("--error-targets", type=int, default=4096)
+    ap.add_argument("--m2l-engine", default="valu", choices=("valu", "mfma"),
+                    help="where the float32 M2L applies its B_U rotation blocks (VALU or matrix cores)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="run one torch copy of this many MiB first (HBM counter calibration)")
     args = ap.parse_args()
@@ -252,12 +254,13 @@ def main() -> int:
     ref_cache = {}
     plain = plain_tuned = None
     if args.mode != "learned-only":
-        plain = measure(FastMultipoleMethod(precision=precision, theta=theta, softening=eps, compensated=True), leaf,
-                        p, "plain")
+        plain = measure(FastMultipoleMethod(precision=precision, theta=theta, softening=eps, compensated=True,
+                                            m2l_engine=args.m2l_engine), leaf, p, "plain")
         tuned = PLAIN_TUNED.get(args.workload)
         if tuned is not None and args.mode == "learned":
             plain_tuned = measure(FastMultipoleMethod(precision=precision, theta=tuned["theta"], softening=eps,
-                                                      compensated=True), tuned["leaf"], tuned["p"], "plain_tuned")
+                                                      compensated=True, m2l_engine=args.m2l_engine),
+                                  tuned["leaf"], tuned["p"], "plain_tuned")
             plain_tuned["config"] = dict(tuned)
     learned = learned246 = None
     if learned_cfg is not None:
@@ -271,7 +274,8 @@ def main() -> int:
             model = load(path)
             return FastMultipoleMethod(precision=precision, theta=cfg["theta"], softening=eps, compensated=True,
                                        mixed_precision=True, adaptive_order=True, p_gears=tuple(model.gears),
-                                       order_model=model, gear_tolerance=cfg["gear_tolerance"])
+                                       order_model=model, gear_tolerance=cfg["gear_tolerance"],
+                                       m2l_engine=args.m2l_engine)
 
         learned = measure(learned_fmm(learned_cfg), learned_cfg["leaf"], max(learned_cfg["gears"]), "learned")
         if args.mode == "learned":
@@ -355,7 +359,7 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": workload, "N": N, "seed": seed,
                    "order": max(learned_cfg["gears"]) if learned is not None else p, "leaf_size": topo_leaf,
-                   "softening": eps, "parallelism": "single-gpu", "mode": mode},
+                   "softening": eps, "parallelism": "single-gpu", "mode": mode, "m2l_engine": args.m2l_engine},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "force_error": head["force_error"],

@@ -248,6 +248,19 @@ int32_t jfm_m2l_gear(void* stream, int32_t dtype, int32_t p, int32_t gear, int32
  * runtime/kernels/_m2l.py:1439-1793), with a mapping that keeps every lane busy
  * when targets have few pairs. */
 int64_t jfm_m2l_flat_scratch_size(int32_t gear, int32_t num_pairs);
+/* Engine of the float32 M2L rotations for every later jfm_m2l / jfm_m2l_gear /
+ * jfm_m2l_flat launch of the process: JFM_M2L_VALU applies the B_U blocks as
+ * unrolled FMAs with the constants folded in, JFM_M2L_MFMA applies them with
+ * v_mfma_f32_4x4x1_16b_f32 (each block's connected components as 4-row
+ * groups; degrees >= 3) while the VALU does the Dz rotations and the z-axis
+ * core.  float64 always runs on the VALU.  The same operator as the reference's
+ * dense rotate -> z-core -> rotate-back (pallas/m2l_real_fused.py:227-277,
+ * B_U per degree from operators/m2l_real_rot_scale.py:161-170); the engine
+ * changes rounding only.  jfm_m2l_engine() returns the current engine. */
+#define JFM_M2L_VALU 0
+#define JFM_M2L_MFMA 1
+int32_t jfm_set_m2l_engine(int32_t engine);
+int32_t jfm_m2l_engine(void);
 int32_t jfm_m2l_flat(void* stream, int32_t dtype, int32_t p, int32_t gear, int32_t num_pairs,
                      const int32_t* pair_tgt, const int32_t* pair_src, const void* multipoles,
                      const void* centers4, void* locals, void* scratch, int32_t num_span,

@@ -25,7 +25,9 @@ EVAL_FAR = 2
 EVAL_POTENTIAL = 4
 EVAL_COMPENSATED = 8
 MAX_ORDER = 8
-ABI_VERSION = 7
+M2L_VALU = 0
+M2L_MFMA = 1
+ABI_VERSION = 8
 
 _i32p = POINTER(c_int32)
 _f64p = POINTER(c_double)
@@ -81,6 +83,8 @@ _SIGNATURES = {
     "jfm_m2l_flat_scratch_size": (c_int64, [c_int32, c_int32]),
     "jfm_m2l_flat": (c_int32, [c_void_p, c_int32, c_int32, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
                                c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p]),
+    "jfm_set_m2l_engine": (c_int32, [c_int32]),
+    "jfm_m2l_engine": (c_int32, []),
     "jfm_l2l_level": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_int32, c_void_p, c_void_p, c_void_p]),
     "jfm_m2m_top": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p,
                               c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
@@ -174,7 +178,7 @@ def call(name: str, *args) -> int:
     rc = getattr(load(), name)(*args)
     if name not in ("jfm_direct_scratch_doubles", "jfm_coeff_stride", "jfm_order_supported", "jfm_abi_version",
                     "jfm_mlp_num_params", "jfm_mlp_backward_scratch_floats", "jfm_tree_bounds_scratch_doubles",
-                    "jfm_key_bits"):
+                    "jfm_key_bits", "jfm_m2l_engine"):
         check(rc, name)
     return rc
 

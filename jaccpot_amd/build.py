@@ -40,6 +40,10 @@ HIP_FLAGS = [
     "-Wno-unknown-warning-option",
 ]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-pthread", "-Wall"]
+# Per-source device flags.  m2l.hip: matrix-instruction results in ordinary
+# VGPRs (the B_U products feed VALU code directly; in AGPRs every result
+# would need a v_accvgpr_read first).
+HIP_FILE_FLAGS = {"m2l.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 
 def _run(cmd, verbose):
@@ -71,7 +75,7 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
     for src in hip_srcs:
         o = obj / (src.stem + ".hip.o")
         objs.append(o)
-        cmds.append([HIPCC, *HIP_FLAGS, *inc, "-c", src, "-o", o])
+        cmds.append([HIPCC, *HIP_FLAGS, *HIP_FILE_FLAGS.get(src.name, []), *inc, "-c", src, "-o", o])
     for src in cpp_srcs:
         o = obj / (src.stem + ".cpp.o")
         objs.append(o)

@@ -12,6 +12,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <type_traits>
+
 #include "device_math.hpp"
 #include "launch.hpp"
 #include "rows.hpp"
@@ -20,15 +23,143 @@ namespace jfm {
 namespace {
 
 constexpr int kWave = 64;
+constexpr int kXcds = 8;
+
+// Workgroups are dealt round-robin to the eight XCDs; this maps launch order
+// to a logical block so that each XCD takes one contiguous run of blocks
+// (targets in tree order: neighbouring targets share most of their far-list
+// sources, which then stay in that XCD's L2).  Bijective: XCD x < r owns
+// q + 1 blocks, the others q, where nblk = 8 q + r.
+__device__ __forceinline__ int xcd_contiguous(int b, int nblk) {
+  const int q = nblk / kXcds, r = nblk % kXcds;
+  const int x = b % kXcds, slot = b / kXcds;
+  return x * q + min(x, r) + slot;
+}
+
 // coefficients per pass of the LDS reductions: a 64 x 33 buffer (8.4 KiB for
 // float32) keeps LDS from limiting the waves resident per CU
 constexpr int kChunk = 32;
+
+// ---------------------------------------------------------------------------
+// Matrix-core form of the B_U applications (float32).
+//
+// Each degree block of B_U (and B_U^T) splits into four connected components
+// of at most five indices (tables_gen.h, kBuGroups).  v_mfma_f32_4x4x1_16b_f32
+// computes sixteen independent 4x4 outer products per instruction: block b
+// (lanes 4b..4b+3) forms D[b][i][j] += A[b][i] * B[b][j] with A[b][i] in lane
+// 4b+i and B[b][j] in lane 4b+j, and lane 4b+j receives D[b][0..3][j] in its
+// four result registers.  With A[b][i] = M[rows[i]][cols[k]] (the same in
+// every block: it depends on lane % 4 only) and B[b][j] = the lane's own
+// coefficient cols[k], a chain of one instruction per component column leaves
+// in every lane the four output rows of its own pair: each lane keeps its
+// pair's coefficient vector in registers, as in the VALU form, and the
+// products run on the matrix pipe while the VALU does the Dz rotations and
+// the z-axis core.  The A operands (85 instructions per matrix at degree 8)
+// sit in LDS, row r of instruction q of matrix t at [(4 t + r) * kAPad + q].
+constexpr int kAPad = ((kBuInstMax + 3) / 4) * 4;
+constexpr int kATable = 8 * kAPad;
+
+struct BuATable {
+  float v[kATable];
+};
+constexpr BuATable make_bu_a_table() {
+  BuATable tab{};
+  for (int t = 0; t < 2; ++t)
+    for (int r = 0; r < 4; ++r)
+      for (int q = 0; q < kAPad; ++q) tab.v[(4 * t + r) * kAPad + q] = q < kBuInstMax ? kBuMfmaA[t][q][r] : 0.0f;
+  return tab;
+}
+__constant__ BuATable kBuATab = make_bu_a_table();
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Copies the A-operand table into LDS (every thread of the block takes part).
+__device__ __forceinline__ void load_bu_a_table(float* lds, int tid, int nthreads) {
+  for (int i = tid; i < kATable; i += nthreads) lds[i] = kBuATab.v[i];
+}
+
+// blk <- M blk for the components GI.. of degree L (M = B_U, or B_U^T if TR);
+// in holds the block's input, at the lane's A row (lds + (lane % 4) * kAPad).
+template <int L, bool TR, int GI>
+__device__ __forceinline__ void bu_mfma_groups(const float* in, float* blk, const float* at) {
+  if constexpr (GI < kBuGroupOff[TR ? 1 : 0][L + 1]) {
+    constexpr BuGroup g = kBuGroups[TR ? 1 : 0][GI];
+    constexpr int base = (TR ? 4 * kAPad : 0) + g.inst0;
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int k = 0; k < g.ncols; ++k) acc = __builtin_amdgcn_mfma_f32_4x4x1f32(at[base + k], in[g.cols[k]], acc, 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (g.rows[i] >= 0) blk[g.rows[i]] = acc[i];
+    bu_mfma_groups<L, TR, GI + 1>(in, blk, at);
+  }
+}
+
+// Degrees below kMfmaMinDegree stay on the VALU: their components are one or
+// two indices, a single FMA each, cheaper than a matrix instruction.
+constexpr int kMfmaMinDegree = 3;
+
+template <int L, bool TR>
+__device__ __forceinline__ void apply_bu_mf(float* blk, const float* at) {
+  if constexpr (L < kMfmaMinDegree) {
+    apply_bu<float, L, TR>(blk);
+  } else {
+    constexpr int W = 2 * L + 1;
+    float in[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) in[j] = blk[j];
+    bu_mfma_groups<L, TR, kBuGroupOff[TR ? 1 : 0][L]>(in, blk, at);
+  }
+}
+
+template <int P, int L, Rot R>
+__device__ __forceinline__ void rotate_block_mf(float* v, const Align<float, P>& a, const float* at) {
+  float* blk = v + L * L;
+  if constexpr (R == Rot::ToZMultipole) {  // B Dz(-ax) B Dz(az)
+    apply_dz<float, L, false>(blk, a.ca, a.sa);
+    apply_bu_mf<L, false>(blk, at);
+    apply_dz<float, L, true>(blk, a.cx, a.sx);
+    apply_bu_mf<L, false>(blk, at);
+  } else {  // FromZLocal: Dz(-az) B^T Dz(ax) B^T
+    static_assert(R == Rot::FromZLocal, "the M2L uses the multipole-to-z and local-from-z rotations");
+    apply_bu_mf<L, true>(blk, at);
+    apply_dz<float, L, false>(blk, a.cx, a.sx);
+    apply_bu_mf<L, true>(blk, at);
+    apply_dz<float, L, true>(blk, a.ca, a.sa);
+  }
+}
+
+template <int P, Rot R, int L = 1>
+__device__ __forceinline__ void rotate_all_mf(float* v, const Align<float, P>& a, const float* at) {
+  if constexpr (L <= P) {
+    rotate_block_mf<P, L, R>(v, a, at);
+    rotate_all_mf<P, R, L + 1>(v, a, at);
+  }
+}
+
+// One pair's M2L contribution w = D_from^L Z(r) D_to^M v at order G (v is
+// overwritten); MF selects the matrix-core B_U (float32 only).
+template <typename T, int G, bool MF>
+__device__ __forceinline__ void m2l_pair(T* v, T* w, T dx, T dy, T dz, const float* at) {
+  Align<T, G> a;
+  a.init(dx, dy, dz);
+  if constexpr (MF) {
+    static_assert(sizeof(T) == 4, "matrix-core B_U is float32 only");
+    rotate_all_mf<G, Rot::ToZMultipole>(v, a, at);
+    zm2l<T, G>(v, w, a.r);
+    rotate_all_mf<G, Rot::FromZLocal>(w, a, at);
+  } else {
+    rotate_all<T, G, Rot::ToZMultipole>(v, a);
+    zm2l<T, G>(v, w, a.r);
+    rotate_all<T, G, Rot::FromZLocal>(w, a);
+  }
+}
 
 // G <= P is the pair's expansion order (gear): the source multipole is read
 // to degree G, translated at order G and added into degrees <= G of the
 // target local.  ACC adds into the existing rows instead of overwriting them
 // (used when a target's pairs are split across several gears).
-template <typename T, int P, int G, bool ACC>
+template <typename T, int P, int G, bool ACC, bool MF>
 __global__ __launch_bounds__(kWave) void m2l_kernel(const int32_t* __restrict__ targets,
                                                     const int32_t* __restrict__ far_off,
                                                     const int32_t* __restrict__ far_src, const T* __restrict__ mult,
@@ -36,26 +167,37 @@ __global__ __launch_bounds__(kWave) void m2l_kernel(const int32_t* __restrict__ 
                                                     int32_t cs) {
   constexpr int C = ncoef(G);
   __shared__ T red[kWave][kChunk + 1];
-  const int tgt = targets[blockIdx.x];
+  __shared__ float atab[MF ? kATable : 1];
+  const int tgt = targets[xcd_contiguous(blockIdx.x, gridDim.x)];
   const int lane = threadIdx.x;
+  if constexpr (MF) {
+    load_bu_a_table(atab, lane, kWave);
+    __syncthreads();
+  }
+  const float* at = atab + (lane & 3) * kAPad;
   const int b = far_off[tgt], e = far_off[tgt + 1];
   const V4<T> ct = load4(centers4, tgt);
   T acc[C];
 #pragma unroll
   for (int j = 0; j < C; ++j) acc[j] = T(0);
-  for (int k = b + lane; k < e; k += kWave) {
-    const int src = far_src[k];
+  // MF: the matrix instructions need every lane of the wave, so the loop runs
+  // to the wave's last round; lanes past the list repeat the target's first
+  // pair and drop the result
+  const int rounds = (e - b + kWave - 1) / kWave;
+  for (int q = 0; q < rounds; ++q) {
+    const int k = b + q * kWave + lane;
+    if (!MF && k >= e) break;
+    const bool live = k < e;
+    const int src = far_src[live ? k : b];
     const V4<T> cs4 = load4(centers4, src);
     T v[C];
     load_row<T, C>(mult + static_cast<int64_t>(src) * cs, v);
-    Align<T, G> a;
-    a.init(ct.x - cs4.x, ct.y - cs4.y, ct.z - cs4.z);  // target - source
-    rotate_all<T, G, Rot::ToZMultipole>(v, a);
     T w[C];
-    zm2l<T, G>(v, w, a.r);
-    rotate_all<T, G, Rot::FromZLocal>(w, a);
+    m2l_pair<T, G, MF>(v, w, ct.x - cs4.x, ct.y - cs4.y, ct.z - cs4.z, at);  // target - source
+    if (live) {
 #pragma unroll
-    for (int j = 0; j < C; ++j) acc[j] += w[j];
+      for (int j = 0; j < C; ++j) acc[j] += w[j];
+    }
   }
   // lane partial sums reduced through LDS, kChunk coefficients at a time
   T* row = locals + static_cast<int64_t>(tgt) * cs;
@@ -90,7 +232,7 @@ __global__ __launch_bounds__(kWave) void m2l_kernel(const int32_t* __restrict__ 
 // and continues), and m2l_fixup_kernel adds those partials in wave order.
 // Deterministic, no atomics; every lane carries a pair whatever the number
 // of pairs per target.
-template <typename T, int P, int G>
+template <typename T, int P, int G, bool MF>
 __global__ __launch_bounds__(kWave) void m2l_flat_kernel(int32_t K, const int32_t* __restrict__ pair_tgt,
                                                          const int32_t* __restrict__ pair_src,
                                                          const T* __restrict__ mult, const T* __restrict__ centers4,
@@ -99,23 +241,35 @@ __global__ __launch_bounds__(kWave) void m2l_flat_kernel(int32_t K, const int32_
   constexpr int C = ncoef(G);
   __shared__ T red[kWave][kChunk + 1];
   __shared__ int32_t tg[kWave];
+  __shared__ float atab[MF ? kATable : 1];
   const int lane = threadIdx.x;
-  const int64_t k0 = static_cast<int64_t>(blockIdx.x) * kWave;
+  if constexpr (MF) {
+    load_bu_a_table(atab, lane, kWave);
+    __syncthreads();
+  }
+  const int blk = xcd_contiguous(blockIdx.x, gridDim.x);  // logical 64-pair block
+  const int64_t k0 = static_cast<int64_t>(blk) * kWave;
   const int64_t k = k0 + lane;
   T w[C];
   int t = -1;
-  if (k < K) {
-    t = pair_tgt[k];
-    const int src = pair_src[k];
-    const V4<T> ct = load4(centers4, t);
+  if (MF || k < K) {
+    // MF: every lane runs the cascade (the matrix instructions need the whole
+    // wave); lanes past the list take pair k0 and contribute nothing
+    const bool live = k < K;
+    const int64_t kk = live ? k : k0;
+    const int tt = pair_tgt[kk];
+    const int src = pair_src[kk];
+    const V4<T> ct = load4(centers4, tt);
     const V4<T> cs4 = load4(centers4, src);
     T v[C];
     load_row<T, C>(mult + static_cast<int64_t>(src) * cs, v);
-    Align<T, G> a;
-    a.init(ct.x - cs4.x, ct.y - cs4.y, ct.z - cs4.z);  // target - source
-    rotate_all<T, G, Rot::ToZMultipole>(v, a);
-    zm2l<T, G>(v, w, a.r);
-    rotate_all<T, G, Rot::FromZLocal>(w, a);
+    m2l_pair<T, G, MF>(v, w, ct.x - cs4.x, ct.y - cs4.y, ct.z - cs4.z, atab + (lane & 3) * kAPad);  // target - source
+    if (live) {
+      t = tt;
+    } else {
+#pragma unroll
+      for (int j = 0; j < C; ++j) w[j] = T(0);
+    }
   } else {
 #pragma unroll
     for (int j = 0; j < C; ++j) w[j] = T(0);
@@ -127,7 +281,7 @@ __global__ __launch_bounds__(kWave) void m2l_flat_kernel(int32_t K, const int32_
   // neighbours of the wave's pair range (uniform)
   const int before = k0 > 0 ? pair_tgt[k0 - 1] : -1;
   const int after = k0 + kWave < K ? pair_tgt[k0 + kWave] : -1;
-  T* part = scratch + static_cast<int64_t>(blockIdx.x) * 2 * C;
+  T* part = scratch + static_cast<int64_t>(blk) * 2 * C;
   // the lanes' contributions go through LDS kChunk coefficients at a time
 #pragma unroll
   for (int c0 = 0; c0 < C; c0 += kChunk) {
@@ -200,6 +354,41 @@ __global__ __launch_bounds__(4 * kWave) void m2l_fixup_kernel(int32_t num_span, 
 
 using namespace jfm;
 
+namespace {
+// Engine of the float32 M2L rotations: 0 = VALU (B_U literals), 1 = matrix
+// cores (v_mfma_f32_4x4x1_16b_f32 for degrees >= kMfmaMinDegree).  float64
+// always runs on the VALU.
+std::atomic<int32_t> g_m2l_engine{JFM_M2L_VALU};
+
+template <typename T>
+bool use_mfma() {
+  return std::is_same<T, float>::value && g_m2l_engine.load(std::memory_order_relaxed) == JFM_M2L_MFMA;
+}
+
+template <typename T, int P, int G, bool ACC>
+void launch_target(hipStream_t s, int32_t nt, const int32_t* targets, const int32_t* off, const int32_t* src,
+                   const void* mult, const void* centers4, void* locals) {
+  const int cs = coef_stride(P, sizeof(T));
+  if constexpr (std::is_same<T, float>::value) {
+    if (use_mfma<T>()) {
+      m2l_kernel<T, P, G, ACC, true><<<nt, kWave, 0, s>>>(targets, off, src, (const T*)mult, (const T*)centers4,
+                                                           (T*)locals, cs);
+      return;
+    }
+  }
+  m2l_kernel<T, P, G, ACC, false><<<nt, kWave, 0, s>>>(targets, off, src, (const T*)mult, (const T*)centers4,
+                                                        (T*)locals, cs);
+}
+}  // namespace
+
+extern "C" int32_t jfm_set_m2l_engine(int32_t engine) {
+  if (engine != JFM_M2L_VALU && engine != JFM_M2L_MFMA) return fail(JFM_ERR_ARG, "jfm_set_m2l_engine: unknown engine");
+  g_m2l_engine.store(engine, std::memory_order_relaxed);
+  return JFM_OK;
+}
+
+extern "C" int32_t jfm_m2l_engine(void) { return g_m2l_engine.load(std::memory_order_relaxed); }
+
 extern "C" int32_t jfm_m2l(void* stream, int32_t dtype, int32_t p, const int32_t* targets, int32_t num_targets,
                            const int32_t* far_off, const int32_t* far_src, const void* multipoles,
                            const void* centers4, void* locals) {
@@ -209,9 +398,7 @@ extern "C" int32_t jfm_m2l(void* stream, int32_t dtype, int32_t p, const int32_t
     return fail(JFM_ERR_ARG, "jfm_m2l: NULL buffer");
   hipStream_t s = as_stream(stream);
   JFM_FOR_DTYPE(dtype, JFM_FOR_ORDER(p, {
-    const int cs = coef_stride(P, sizeof(T));
-    m2l_kernel<T, P, P, false><<<num_targets, kWave, 0, s>>>(targets, far_off, far_src, (const T*)multipoles,
-                                                             (const T*)centers4, (T*)locals, cs);
+    launch_target<T, P, P, false>(s, num_targets, targets, far_off, far_src, multipoles, centers4, locals);
   }));
   return check_launch("jfm_m2l");
 }
@@ -220,18 +407,15 @@ namespace {
 template <typename T, int P>
 int32_t launch_gear(hipStream_t s, int32_t gear, bool acc, int32_t nt, const int32_t* targets, const int32_t* off,
                     const int32_t* src, const void* mult, const void* centers4, void* locals) {
-  const int cs = coef_stride(P, sizeof(T));
-#define JFM_GEAR(G)                                                                                              \
-  if (gear == G) {                                                                                               \
-    if constexpr (G <= P) {                                                                                      \
-      if (acc)                                                                                                   \
-        m2l_kernel<T, P, G, true><<<nt, kWave, 0, s>>>(targets, off, src, (const T*)mult, (const T*)centers4,    \
-                                                       (T*)locals, cs);                                          \
-      else                                                                                                       \
-        m2l_kernel<T, P, G, false><<<nt, kWave, 0, s>>>(targets, off, src, (const T*)mult, (const T*)centers4,   \
-                                                        (T*)locals, cs);                                         \
-      return check_launch("jfm_m2l_gear");                                                                       \
-    }                                                                                                            \
+#define JFM_GEAR(G)                                                                                \
+  if (gear == G) {                                                                                 \
+    if constexpr (G <= P) {                                                                        \
+      if (acc)                                                                                     \
+        launch_target<T, P, G, true>(s, nt, targets, off, src, mult, centers4, locals);            \
+      else                                                                                         \
+        launch_target<T, P, G, false>(s, nt, targets, off, src, mult, centers4, locals);           \
+      return check_launch("jfm_m2l_gear");                                                         \
+    }                                                                                              \
   }
   JFM_GEAR(1) JFM_GEAR(2) JFM_GEAR(3) JFM_GEAR(4) JFM_GEAR(5) JFM_GEAR(6) JFM_GEAR(7) JFM_GEAR(8)
 #undef JFM_GEAR
@@ -262,6 +446,21 @@ extern "C" int64_t jfm_m2l_flat_scratch_size(int32_t gear, int32_t num_pairs) {
 }
 
 namespace {
+template <typename T, int P, int G>
+void launch_flat_pairs(hipStream_t s, int nw, int32_t K, const int32_t* pair_tgt, const int32_t* pair_src,
+                       const void* mult, const void* centers4, void* locals, void* scratch) {
+  const int cs = coef_stride(P, sizeof(T));
+  if constexpr (std::is_same<T, float>::value) {
+    if (use_mfma<T>()) {
+      m2l_flat_kernel<T, P, G, true><<<nw, kWave, 0, s>>>(K, pair_tgt, pair_src, (const T*)mult,
+                                                           (const T*)centers4, (T*)locals, cs, (T*)scratch);
+      return;
+    }
+  }
+  m2l_flat_kernel<T, P, G, false><<<nw, kWave, 0, s>>>(K, pair_tgt, pair_src, (const T*)mult, (const T*)centers4,
+                                                        (T*)locals, cs, (T*)scratch);
+}
+
 template <typename T, int P>
 int32_t launch_flat(hipStream_t s, int32_t gear, int32_t K, const int32_t* pair_tgt, const int32_t* pair_src,
                     const void* mult, const void* centers4, void* locals, void* scratch, int32_t num_span,
@@ -271,8 +470,7 @@ int32_t launch_flat(hipStream_t s, int32_t gear, int32_t K, const int32_t* pair_
 #define JFM_FLAT(G)                                                                                              \
   if (gear == G) {                                                                                               \
     if constexpr (G <= P) {                                                                                      \
-      m2l_flat_kernel<T, P, G><<<nw, kWave, 0, s>>>(K, pair_tgt, pair_src, (const T*)mult, (const T*)centers4,   \
-                                                    (T*)locals, cs, (T*)scratch);                                \
+      launch_flat_pairs<T, P, G>(s, nw, K, pair_tgt, pair_src, mult, centers4, locals, scratch);                 \
       if (num_span > 0)                                                                                          \
         m2l_fixup_kernel<T, G><<<(num_span + 3) / 4, 4 * kWave, 0, s>>>(num_span, span_tgt, span_w0, span_w1,   \
                                                                         (const T*)scratch, (T*)locals, cs);      \

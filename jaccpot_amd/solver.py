@@ -106,7 +106,10 @@ class FastMultipoleMethod:
     (C++ threads); both give the same arrays bit for bit.  ``split_near``
     (default on) cuts long near lists into P2P work items of at most 8192
     source slots (``nearsplit``), so a few leaves with huge near lists do not
-    run alone at the end of the P2P launch.  ``reuse_topology`` with
+    run alone at the end of the P2P launch.  ``m2l_engine`` picks where the
+    float32 M2L applies its constant B_U rotation blocks: ``"valu"`` (unrolled
+    FMAs with the constants folded in) or ``"mfma"`` (matrix cores,
+    ``v_mfma_f32_4x4x1_16b_f32``); float64 always uses the VALU.  ``reuse_topology`` with
     ``rebuild_every`` (solver.py:592-597) keeps the tree and the far/near lists
     of the last ``prepare_state`` for up to ``rebuild_every - 1`` further calls
     whose sorted Morton codes are unchanged (runtime/fmm_prepare.py:797-802):
@@ -148,6 +151,7 @@ class FastMultipoleMethod:
         num_threads: int = 0,
         tree_build: str = "device",
         split_near: bool = True,
+        m2l_engine: str = "valu",
         **legacy_kwargs: Any,
     ):
         from .config import resolve_constructor_options
@@ -185,6 +189,8 @@ class FastMultipoleMethod:
             raise ValueError("tree_build must be 'device' or 'host'")
         if int(rebuild_every) <= 0:
             raise ValueError("rebuild_every must be positive")
+        if m2l_engine not in ("valu", "mfma"):
+            raise ValueError("m2l_engine must be 'valu' or 'mfma'")
         if int(order_judge_targets) < 0 or not (float(order_judge_factor) >= 1.0):
             raise ValueError("order_judge_targets must be >= 0 and order_judge_factor >= 1")
         dtype = None
@@ -219,6 +225,7 @@ class FastMultipoleMethod:
         self.num_threads = int(num_threads)
         self.tree_build = tree_build
         self.split_near = bool(split_near)
+        self.m2l_engine = m2l_engine
         self.reuse_topology = bool(reuse_topology)
         self.rebuild_every = int(rebuild_every)
         self._reuse: Optional[dict] = None
@@ -514,6 +521,8 @@ class FastMultipoleMethod:
     def _downward(self, state: FMMPreparedState, s, code) -> None:
         topo = state.topology
         state.locals.zero_()
+        # library-wide engine of the float32 M2L rotations, set before every downward pass
+        nat.call("jfm_set_m2l_engine", nat.M2L_MFMA if self.m2l_engine == "mfma" else nat.M2L_VALU)
         sched = state.extra.get("gears")
         if sched is not None:
             for i, g in enumerate(sched.gears):
