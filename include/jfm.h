@@ -452,6 +452,20 @@ int64_t jfm_direct_scratch_doubles(int32_t nt);
 int32_t jfm_direct_sum_f64(void* stream, const double* pos4, int32_t n, const int32_t* targets, int32_t nt,
                            double eps2, double G, double* scratch, double* acc, double* pot);
 
+/* jfm_direct_sum_f64 without a scratch argument and without any allocation:
+ * the per-chunk partials live in a fixed per-GPU device array (32 MiB, held
+ * by each GPU the library runs on until the process ends), so at most
+ * jfm_direct_sum_f64_static_max_targets() = 4096 targets per call; more is
+ * refused with JFM_ERR_ARG.  Same kernels, same summation order, same result
+ * bit for bit as jfm_direct_sum_f64.  The array belongs to the current device
+ * (the stream must be on it); every call on a GPU reuses it, so calls that
+ * overlap on one GPU (other streams, other threads) race.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc). */
+int32_t jfm_direct_sum_f64_static_max_targets(void);
+int32_t jfm_direct_sum_f64_static(void* stream, const double* pos4, int32_t n, const int32_t* targets, int32_t nt,
+                                  double eps2, double G, double* acc, double* pot);
+
 /* ------------------------------------------------------------------------
  * Validation only: host (CPU) instantiation of the kernels' per-pair operator
  * math, one pair per call, for checking against the float64 oracle without a

@@ -135,6 +135,9 @@ _SIGNATURES = {
     "jfm_select_gears": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_float,
                                    c_void_p, c_void_p]),
     "jfm_host_operator": (c_int32, [c_int32, c_int32, c_int32, c_void_p, c_void_p, c_void_p]),
+    "jfm_direct_sum_f64_static_max_targets": (c_int32, []),
+    "jfm_direct_sum_f64_static": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_double, c_double,
+                                            c_void_p, c_void_p]),
     "jfm_direct_sum_f64": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_double, c_double, c_void_p,
                                      c_void_p, c_void_p]),
 }
@@ -178,7 +181,7 @@ def call(name: str, *args) -> int:
     rc = getattr(load(), name)(*args)
     if name not in ("jfm_direct_scratch_doubles", "jfm_coeff_stride", "jfm_order_supported", "jfm_abi_version",
                     "jfm_mlp_num_params", "jfm_mlp_backward_scratch_floats", "jfm_tree_bounds_scratch_doubles",
-                    "jfm_key_bits", "jfm_m2l_engine"):
+                    "jfm_key_bits", "jfm_m2l_engine", "jfm_direct_sum_f64_static_max_targets"):
         check(rc, name)
     return rc
 

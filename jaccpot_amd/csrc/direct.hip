@@ -3,10 +3,13 @@
 // Newtonian sum, self term masked).
 //
 // Grid: (target blocks of 256) x (source chunks).  Each workgroup streams its
-// source chunk through LDS; per-chunk partials go to caller scratch and a
-// second kernel reduces them in chunk order, so the result is deterministic.
+// source chunk through LDS; per-chunk partials go to scratch (the caller's, or
+// for jfm_direct_sum_f64_static a fixed per-GPU array) and a second kernel
+// reduces them in chunk order, so the result is deterministic.
 
 #include <hip/hip_runtime.h>
+
+#include <mutex>
 
 #include "launch.hpp"
 #include "rows.hpp"
@@ -70,6 +73,47 @@ __global__ __launch_bounds__(kThreads) void direct_reduce_kernel(const double* _
   if (pot) pot[t] = -G * ap;
 }
 
+// Scratch of jfm_direct_sum_f64_static: the per-chunk partials of up to
+// kStaticTargets targets (kChunks * 4 * kStaticTargets doubles = 32 MiB),
+// one copy per GPU, held until the process ends.
+constexpr int32_t kStaticTargets = 4096;
+__device__ __align__(256) double scratch[static_cast<int64_t>(kChunks) * 4 * kStaticTargets];
+
+// Device address of `scratch` on each GPU, looked up once per device with that
+// device current (host code never passes the symbol itself to a kernel).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+int32_t static_scratch(double** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(JFM_ERR_HIP, std::string("jfm_direct_sum_f64_static: hipGetDevice: ") +
+                                                   hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return fail(JFM_ERR_ARG, "jfm_direct_sum_f64_static: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(JFM_ERR_HIP, std::string("jfm_direct_sum_f64_static: hipGetSymbolAddress: ") +
+                                   hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return JFM_OK;
+}
+
+int32_t launch_direct(hipStream_t s, const double* pos4, int32_t n, const int32_t* targets, int32_t nt, double eps2,
+                      double G, double* part, double* acc, double* pot, const char* what) {
+  dim3 grid((nt + kThreads - 1) / kThreads, kChunks);
+  direct_partial_kernel<<<grid, kThreads, 0, s>>>(pos4, n, targets, nt, eps2, part);
+  int32_t rc = check_launch(what);
+  if (rc != JFM_OK) return rc;
+  direct_reduce_kernel<<<(nt + kThreads - 1) / kThreads, kThreads, 0, s>>>(part, nt, G, acc, pot);
+  return check_launch(what);
+}
+
 }  // namespace
 }  // namespace jfm
 
@@ -86,13 +130,26 @@ int32_t jfm_direct_sum_f64(void* stream, const double* pos4, int32_t n, const in
   if (!pos4 || !targets || !scratch || !acc) return fail(JFM_ERR_ARG, "jfm_direct_sum_f64: NULL buffer");
   if (static_cast<int64_t>(kChunks) * nt > (int64_t(1) << 31) / 4)
     return fail(JFM_ERR_ARG, "jfm_direct_sum_f64: too many targets");
-  hipStream_t s = as_stream(stream);
-  dim3 grid((nt + kThreads - 1) / kThreads, kChunks);
-  direct_partial_kernel<<<grid, kThreads, 0, s>>>(pos4, n, targets, nt, eps2, scratch);
-  int32_t rc = check_launch("jfm_direct_sum_f64");
+  return launch_direct(as_stream(stream), pos4, n, targets, nt, eps2, G, scratch, acc, pot, "jfm_direct_sum_f64");
+}
+
+int32_t jfm_direct_sum_f64_static_max_targets(void) { return kStaticTargets; }
+
+int32_t jfm_direct_sum_f64_static(void* stream, const double* pos4, int32_t n, const int32_t* targets, int32_t nt,
+                                  double eps2, double G, double* acc, double* pot) {
+  if (n < 1 || nt < 0) return fail(JFM_ERR_ARG, "jfm_direct_sum_f64_static: bad sizes");
+  if (nt > kStaticTargets)
+    return fail(JFM_ERR_ARG, "jfm_direct_sum_f64_static: " + std::to_string(nt) + " targets exceed the maximum of " +
+                                 std::to_string(kStaticTargets));
+  if (nt == 0) return JFM_OK;
+  if (!pos4 || !targets || !acc) return fail(JFM_ERR_ARG, "jfm_direct_sum_f64_static: NULL buffer");
+  double* part = nullptr;
+  const int32_t rc = static_scratch(&part);
   if (rc != JFM_OK) return rc;
-  direct_reduce_kernel<<<(nt + kThreads - 1) / kThreads, kThreads, 0, s>>>(scratch, nt, G, acc, pot);
-  return check_launch("jfm_direct_sum_f64");
+  // every partial the reduction reads is written by this call's first kernel,
+  // so the previous call's contents need no clearing
+  return launch_direct(as_stream(stream), pos4, n, targets, nt, eps2, G, part, acc, pot,
+                       "jfm_direct_sum_f64_static");
 }
 
 }  // extern "C"
