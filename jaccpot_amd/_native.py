@@ -27,6 +27,7 @@ EVAL_COMPENSATED = 8
 MAX_ORDER = 8
 M2L_VALU = 0
 M2L_MFMA = 1
+M2L_ENGINES = {"valu": M2L_VALU, "mfma": M2L_MFMA}
 ABI_VERSION = 8
 
 _i32p = POINTER(c_int32)

@@ -41,57 +41,76 @@ __device__ __forceinline__ int xcd_contiguous(int b, int nblk) {
 constexpr int kChunk = 32;
 
 // ---------------------------------------------------------------------------
-// Matrix-core form of the B_U applications (float32).
+// Matrix-core form of the B_U applications (float32; m2l_engine "mfma").
 //
 // Each degree block of B_U (and B_U^T) splits into four connected components
 // of at most five indices (tables_gen.h, kBuGroups).  v_mfma_f32_4x4x1_16b_f32
 // computes sixteen independent 4x4 outer products per instruction: block b
-// (lanes 4b..4b+3) forms D[b][i][j] += A[b][i] * B[b][j] with A[b][i] in lane
-// 4b+i and B[b][j] in lane 4b+j, and lane 4b+j receives D[b][0..3][j] in its
-// four result registers.  With A[b][i] = M[rows[i]][cols[k]] (the same in
-// every block: it depends on lane % 4 only) and B[b][j] = the lane's own
-// coefficient cols[k], a chain of one instruction per component column leaves
-// in every lane the four output rows of its own pair: each lane keeps its
-// pair's coefficient vector in registers, as in the VALU form, and the
-// products run on the matrix pipe while the VALU does the Dz rotations and
-// the z-axis core.  The A operands (85 instructions per matrix at degree 8)
-// sit in LDS, row r of instruction q of matrix t at [(4 t + r) * kAPad + q].
-constexpr int kAPad = ((kBuInstMax + 3) / 4) * 4;
-constexpr int kATable = 8 * kAPad;
-
-struct BuATable {
-  float v[kATable];
-};
-constexpr BuATable make_bu_a_table() {
-  BuATable tab{};
-  for (int t = 0; t < 2; ++t)
-    for (int r = 0; r < 4; ++r)
-      for (int q = 0; q < kAPad; ++q) tab.v[(4 * t + r) * kAPad + q] = q < kBuInstMax ? kBuMfmaA[t][q][r] : 0.0f;
-  return tab;
-}
-__constant__ BuATable kBuATab = make_bu_a_table();
-
+// (lanes 4b..4b+3) forms D[b][i][j] += A[b][i] * B[b][j] with B[b][j] in lane
+// 4b+j, and lane 4b+j receives D[b][0..3][j] in its four result registers.
+// With A[b][i] = M[rows[i]][cols[k]] (the same in every block) and B[b][j] =
+// the lane's own coefficient cols[k], a chain of one instruction per
+// component column leaves in every lane the four output rows of its own pair:
+// each lane keeps its pair's coefficient vector in registers, as in the VALU
+// form, and the products run on the matrix pipe while the VALU does the Dz
+// rotations and the z-axis core.
+//
+// The A operands are broadcast: with CBSZ = 4 the instruction takes the A
+// values of block ABID (lanes 4 ABID .. 4 ABID + 3) for all sixteen blocks,
+// so one VGPR carries the A operands of sixteen instructions.  Instruction q
+// of matrix t reads register kBuRegs * t + q / 16 with ABID = q % 16, lane
+// 4 (q % 16) + r holding row r; the twelve registers (85 instructions per
+// matrix at degree 8) are loaded once per wave.
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// Copies the A-operand table into LDS (every thread of the block takes part).
-__device__ __forceinline__ void load_bu_a_table(float* lds, int tid, int nthreads) {
-  for (int i = tid; i < kATable; i += nthreads) lds[i] = kBuATab.v[i];
+constexpr int kBuRegs = (kBuInstMax + 15) / 16;
+struct BuBcastTable {
+  float v[2 * kBuRegs * 64];
+};
+constexpr BuBcastTable make_bu_bcast_table() {
+  BuBcastTable tab{};
+  for (int t = 0; t < 2; ++t)
+    for (int g = 0; g < kBuRegs; ++g)
+      for (int l = 0; l < 64; ++l) {
+        const int q = 16 * g + l / 4;
+        tab.v[(t * kBuRegs + g) * 64 + l] = q < kBuInstMax ? kBuMfmaA[t][q][l % 4] : 0.0f;
+      }
+  return tab;
+}
+__constant__ BuBcastTable kBuBcast = make_bu_bcast_table();
+
+// This lane's A registers (kBuBcast rows of lane `lane`).
+struct BuRegs {
+  float r[2 * kBuRegs];
+  __device__ __forceinline__ void load(int lane) {
+#pragma unroll
+    for (int q = 0; q < 2 * kBuRegs; ++q) r[q] = kBuBcast.v[q * 64 + lane];
+  }
+};
+
+// Columns K.. of component GI (instruction q = g.inst0 + K).
+template <bool TR, int GI, int K>
+__device__ __forceinline__ void bu_mfma_cols(const float* in, f32x4& acc, const BuRegs& a) {
+  constexpr BuGroup g = kBuGroups[TR ? 1 : 0][GI];
+  if constexpr (K < g.ncols) {
+    constexpr int q = g.inst0 + K;
+    acc = __builtin_amdgcn_mfma_f32_4x4x1f32(a.r[(TR ? kBuRegs : 0) + q / 16], in[g.cols[K]], acc, 4, q % 16, 0);
+    bu_mfma_cols<TR, GI, K + 1>(in, acc, a);
+  }
 }
 
 // blk <- M blk for the components GI.. of degree L (M = B_U, or B_U^T if TR);
-// in holds the block's input, at the lane's A row (lds + (lane % 4) * kAPad).
+// in holds the block's input.
 template <int L, bool TR, int GI>
-__device__ __forceinline__ void bu_mfma_groups(const float* in, float* blk, const float* at) {
+__device__ __forceinline__ void bu_mfma_groups(const float* in, float* blk, const BuRegs& a) {
   if constexpr (GI < kBuGroupOff[TR ? 1 : 0][L + 1]) {
     constexpr BuGroup g = kBuGroups[TR ? 1 : 0][GI];
-    constexpr int base = (TR ? 4 * kAPad : 0) + g.inst0;
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int k = 0; k < g.ncols; ++k) acc = __builtin_amdgcn_mfma_f32_4x4x1f32(at[base + k], in[g.cols[k]], acc, 0, 0, 0);
+    bu_mfma_cols<TR, GI, 0>(in, acc, a);
 #pragma unroll
     for (int i = 0; i < 4; ++i)
       if (g.rows[i] >= 0) blk[g.rows[i]] = acc[i];
-    bu_mfma_groups<L, TR, GI + 1>(in, blk, at);
+    bu_mfma_groups<L, TR, GI + 1>(in, blk, a);
   }
 }
 
@@ -100,7 +119,7 @@ __device__ __forceinline__ void bu_mfma_groups(const float* in, float* blk, cons
 constexpr int kMfmaMinDegree = 3;
 
 template <int L, bool TR>
-__device__ __forceinline__ void apply_bu_mf(float* blk, const float* at) {
+__device__ __forceinline__ void apply_bu_mf(float* blk, const BuRegs& a) {
   if constexpr (L < kMfmaMinDegree) {
     apply_bu<float, L, TR>(blk);
   } else {
@@ -108,46 +127,46 @@ __device__ __forceinline__ void apply_bu_mf(float* blk, const float* at) {
     float in[W];
 #pragma unroll
     for (int j = 0; j < W; ++j) in[j] = blk[j];
-    bu_mfma_groups<L, TR, kBuGroupOff[TR ? 1 : 0][L]>(in, blk, at);
+    bu_mfma_groups<L, TR, kBuGroupOff[TR ? 1 : 0][L]>(in, blk, a);
   }
 }
 
 template <int P, int L, Rot R>
-__device__ __forceinline__ void rotate_block_mf(float* v, const Align<float, P>& a, const float* at) {
+__device__ __forceinline__ void rotate_block_mf(float* v, const Align<float, P>& al, const BuRegs& a) {
   float* blk = v + L * L;
   if constexpr (R == Rot::ToZMultipole) {  // B Dz(-ax) B Dz(az)
-    apply_dz<float, L, false>(blk, a.ca, a.sa);
-    apply_bu_mf<L, false>(blk, at);
-    apply_dz<float, L, true>(blk, a.cx, a.sx);
-    apply_bu_mf<L, false>(blk, at);
+    apply_dz<float, L, false>(blk, al.ca, al.sa);
+    apply_bu_mf<L, false>(blk, a);
+    apply_dz<float, L, true>(blk, al.cx, al.sx);
+    apply_bu_mf<L, false>(blk, a);
   } else {  // FromZLocal: Dz(-az) B^T Dz(ax) B^T
     static_assert(R == Rot::FromZLocal, "the M2L uses the multipole-to-z and local-from-z rotations");
-    apply_bu_mf<L, true>(blk, at);
-    apply_dz<float, L, false>(blk, a.cx, a.sx);
-    apply_bu_mf<L, true>(blk, at);
-    apply_dz<float, L, true>(blk, a.ca, a.sa);
+    apply_bu_mf<L, true>(blk, a);
+    apply_dz<float, L, false>(blk, al.cx, al.sx);
+    apply_bu_mf<L, true>(blk, a);
+    apply_dz<float, L, true>(blk, al.ca, al.sa);
   }
 }
 
 template <int P, Rot R, int L = 1>
-__device__ __forceinline__ void rotate_all_mf(float* v, const Align<float, P>& a, const float* at) {
+__device__ __forceinline__ void rotate_all_mf(float* v, const Align<float, P>& al, const BuRegs& a) {
   if constexpr (L <= P) {
-    rotate_block_mf<P, L, R>(v, a, at);
-    rotate_all_mf<P, R, L + 1>(v, a, at);
+    rotate_block_mf<P, L, R>(v, al, a);
+    rotate_all_mf<P, R, L + 1>(v, al, a);
   }
 }
 
 // One pair's M2L contribution w = D_from^L Z(r) D_to^M v at order G (v is
-// overwritten); MF selects the matrix-core B_U (float32 only).
+// overwritten); MF: B_U on the matrix cores (float32 only), else the VALU.
 template <typename T, int G, bool MF>
-__device__ __forceinline__ void m2l_pair(T* v, T* w, T dx, T dy, T dz, const float* at) {
+__device__ __forceinline__ void m2l_pair(T* v, T* w, T dx, T dy, T dz, const BuRegs& bu) {
   Align<T, G> a;
   a.init(dx, dy, dz);
   if constexpr (MF) {
     static_assert(sizeof(T) == 4, "matrix-core B_U is float32 only");
-    rotate_all_mf<G, Rot::ToZMultipole>(v, a, at);
+    rotate_all_mf<G, Rot::ToZMultipole>(v, a, bu);
     zm2l<T, G>(v, w, a.r);
-    rotate_all_mf<G, Rot::FromZLocal>(w, a, at);
+    rotate_all_mf<G, Rot::FromZLocal>(w, a, bu);
   } else {
     rotate_all<T, G, Rot::ToZMultipole>(v, a);
     zm2l<T, G>(v, w, a.r);
@@ -167,14 +186,10 @@ __global__ __launch_bounds__(kWave) void m2l_kernel(const int32_t* __restrict__ 
                                                     int32_t cs) {
   constexpr int C = ncoef(G);
   __shared__ T red[kWave][kChunk + 1];
-  __shared__ float atab[MF ? kATable : 1];
   const int tgt = targets[xcd_contiguous(blockIdx.x, gridDim.x)];
   const int lane = threadIdx.x;
-  if constexpr (MF) {
-    load_bu_a_table(atab, lane, kWave);
-    __syncthreads();
-  }
-  const float* at = atab + (lane & 3) * kAPad;
+  BuRegs bu;
+  if constexpr (MF) bu.load(lane);
   const int b = far_off[tgt], e = far_off[tgt + 1];
   const V4<T> ct = load4(centers4, tgt);
   T acc[C];
@@ -193,7 +208,7 @@ __global__ __launch_bounds__(kWave) void m2l_kernel(const int32_t* __restrict__ 
     T v[C];
     load_row<T, C>(mult + static_cast<int64_t>(src) * cs, v);
     T w[C];
-    m2l_pair<T, G, MF>(v, w, ct.x - cs4.x, ct.y - cs4.y, ct.z - cs4.z, at);  // target - source
+    m2l_pair<T, G, MF>(v, w, ct.x - cs4.x, ct.y - cs4.y, ct.z - cs4.z, bu);  // target - source
     if (live) {
 #pragma unroll
       for (int j = 0; j < C; ++j) acc[j] += w[j];
@@ -241,12 +256,9 @@ __global__ __launch_bounds__(kWave) void m2l_flat_kernel(int32_t K, const int32_
   constexpr int C = ncoef(G);
   __shared__ T red[kWave][kChunk + 1];
   __shared__ int32_t tg[kWave];
-  __shared__ float atab[MF ? kATable : 1];
   const int lane = threadIdx.x;
-  if constexpr (MF) {
-    load_bu_a_table(atab, lane, kWave);
-    __syncthreads();
-  }
+  BuRegs bu;
+  if constexpr (MF) bu.load(lane);
   const int blk = xcd_contiguous(blockIdx.x, gridDim.x);  // logical 64-pair block
   const int64_t k0 = static_cast<int64_t>(blk) * kWave;
   const int64_t k = k0 + lane;
@@ -263,7 +275,7 @@ __global__ __launch_bounds__(kWave) void m2l_flat_kernel(int32_t K, const int32_
     const V4<T> cs4 = load4(centers4, src);
     T v[C];
     load_row<T, C>(mult + static_cast<int64_t>(src) * cs, v);
-    m2l_pair<T, G, MF>(v, w, ct.x - cs4.x, ct.y - cs4.y, ct.z - cs4.z, atab + (lane & 3) * kAPad);  // target - source
+    m2l_pair<T, G, MF>(v, w, ct.x - cs4.x, ct.y - cs4.y, ct.z - cs4.z, bu);  // target - source
     if (live) {
       t = tt;
     } else {

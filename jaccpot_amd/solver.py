@@ -108,8 +108,9 @@ class FastMultipoleMethod:
     source slots (``nearsplit``), so a few leaves with huge near lists do not
     run alone at the end of the P2P launch.  ``m2l_engine`` picks where the
     float32 M2L applies its constant B_U rotation blocks: ``"valu"`` (unrolled
-    FMAs with the constants folded in) or ``"mfma"`` (matrix cores,
-    ``v_mfma_f32_4x4x1_16b_f32``); float64 always uses the VALU.  ``reuse_topology`` with
+    FMAs with the constants folded in; the default, faster on MI355X) or
+    ``"mfma"`` (matrix cores, ``v_mfma_f32_4x4x1_16b_f32``); float64 always
+    uses the VALU.  ``reuse_topology`` with
     ``rebuild_every`` (solver.py:592-597) keeps the tree and the far/near lists
     of the last ``prepare_state`` for up to ``rebuild_every - 1`` further calls
     whose sorted Morton codes are unchanged (runtime/fmm_prepare.py:797-802):
@@ -522,7 +523,7 @@ class FastMultipoleMethod:
         topo = state.topology
         state.locals.zero_()
         # library-wide engine of the float32 M2L rotations, set before every downward pass
-        nat.call("jfm_set_m2l_engine", nat.M2L_MFMA if self.m2l_engine == "mfma" else nat.M2L_VALU)
+        nat.call("jfm_set_m2l_engine", nat.M2L_ENGINES[self.m2l_engine])
         sched = state.extra.get("gears")
         if sched is not None:
             for i, g in enumerate(sched.gears):

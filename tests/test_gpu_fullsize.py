@@ -197,8 +197,11 @@ def test_blobs_1m_gradient_protocol(require_gpu):
     print(f"[gpu]   central differences h={h} over 64 directions: max rel {rel.max():.3e} median {np.median(rel):.3e}; "
           f"Richardson (h, h/2) on {len(rich)} of them: max rel {rel_r.max():.3e} (same directions at h: "
           f"{rel_h.max():.3e})", flush=True)
-    # the reference's FD-vs-AD bound (tests/unit/test_gradient_correctness.py:73-88)
+    # the reference's FD-vs-AD bound (tests/unit/test_gradient_correctness.py:73-88); the O(h^2)
+    # term of the central difference is ~3e-6 here, and once it is extrapolated away the
+    # adjoint agrees to ~3e-10
     assert rel.max() < 1e-4
+    assert rel_r.max() < 1e-8
 
 
 @pytest.mark.timeout(600)

@@ -97,6 +97,6 @@ def test_mfma_engine_flat_gears_match_cpu_restatement(require_gpu):
 def test_engine_switch_rejects_unknown_engine(require_gpu):
     from jaccpot_amd import _native as nat
 
-    assert nat.call("jfm_m2l_engine") in (nat.M2L_VALU, nat.M2L_MFMA)
+    assert nat.call("jfm_m2l_engine") in nat.M2L_ENGINES.values()
     with pytest.raises(nat.JfmError):
         nat.call("jfm_set_m2l_engine", 7)

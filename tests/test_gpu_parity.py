@@ -244,7 +244,9 @@ def test_flat_m2l_matches_per_target_m2l(dtype_name, require_gpu):
     scale = np.maximum(np.abs(b).max(axis=1, keepdims=True), 1e-300)
     err = float((np.abs(a - b) / scale).max())
     print(f"[gpu]   max row-relative difference {err:.3e}", flush=True)
-    assert err < (1e-12 if dtype_name == "fp64" else 2e-6)
+    # float32: the two kernels add up to several hundred pair contributions per row in different
+    # orders (the per-target kernel by lane, then across lanes; the flat one in 64-pair blocks)
+    assert err < (1e-12 if dtype_name == "fp64" else 5e-6)
 
 
 @pytest.mark.parametrize("learned", [False, True], ids=["plain", "learned_mixed"])
