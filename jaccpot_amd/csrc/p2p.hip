@@ -180,32 +180,50 @@ struct P2PItems {
   int32_t num_part;  // rows of partials (items with part >= num_part write nothing)
 };
 
+// Workgroup size of the float32 leaf-granular path for leaves of 8: 128
+// threads (16 source partitions), so each thread runs 64 interactions per
+// 1024-source tile and the per-item set-up and partition reduction are shared
+// by half as many threads as with 256.
+#ifndef JFM_P2P_NARROW_THREADS
+#define JFM_P2P_NARROW_THREADS 128
+#endif
+constexpr int kNarrowThreads = JFM_P2P_NARROW_THREADS;
+// Narrow workgroups per CU (LDS reservation, as for the 256-thread form): 8 x
+// 2 waves, so their registers leave room on every SIMD for a tree-pass wave.
+#ifndef JFM_P2P_NARROW_BLOCKS_PER_CU
+#define JFM_P2P_NARROW_BLOCKS_PER_CU 8
+#endif
+
 // SAFE: guard r^2 == 0 (unsoftened self term / coincident points).
-template <typename T, int WT, bool POT, bool COMP, bool SAFE, bool FIXED>
-__global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos4, int32_t n, int32_t leaf_size,
-                                                       const int32_t* __restrict__ near_off,
-                                                       const int32_t* __restrict__ near_src, T eps2, T G,
-                                                       const int32_t* __restrict__ out_index, T* __restrict__ acc,
-                                                       T* __restrict__ pot, int32_t accumulate, P2PItems<T> items) {
-  constexpr int NPART = kThreads / WT;
+// THREADS: workgroup size (kThreads, or kNarrowThreads for narrow leaves).
+template <typename T, int WT, bool POT, bool COMP, bool SAFE, bool FIXED, int THREADS>
+__global__ __launch_bounds__(THREADS) void p2p_kernel(const T* __restrict__ pos4, int32_t n, int32_t leaf_size,
+                                                      const int32_t* __restrict__ near_off,
+                                                      const int32_t* __restrict__ near_src, T eps2, T G,
+                                                      const int32_t* __restrict__ out_index, T* __restrict__ acc,
+                                                      T* __restrict__ pot, int32_t accumulate, P2PItems<T> items) {
+  constexpr int NPART = THREADS / WT;
   constexpr int TILE = sizeof(T) == 4 ? 1024 : 512;
-  constexpr int LOADS = TILE / kThreads;
-  static_assert(2 * kThreads <= TILE, "reduction buffer must fit in the tile");
+  constexpr int LOADS = TILE / THREADS;
+  static_assert(2 * THREADS <= TILE, "reduction buffer must fit in the tile");
   // REMAP: leaf-granular fetch; slot s of a tile is stored at
   // (s % LOADS) * ROW + s / LOADS, so the fetching thread writes its LOADS
   // slots to LOADS conflict-free rows and partition p still reads one address
   // per interaction, (p % LOADS) * ROW + p / LOADS + k * NPART / LOADS.  Rows
-  // are padded by two elements so the partitions a wave reads (p and p + 1
-  // land LOADS rows apart otherwise) fall in distinct LDS banks.
+  // are padded so that the eight partitions a wave reads fall in distinct
+  // 4-bank groups: with LOADS = 4 they sit in rows p % 4 at columns p / 4
+  // (ROW = THREADS + 2), with LOADS = 8 in rows p at column 0 (ROW = THREADS + 1).
   constexpr bool REMAP = FIXED && (NPART % LOADS == 0);
   constexpr bool BUF = REMAP && sizeof(T) == 4;  // float32: buffer-resource fetch
-  constexpr int ROW = REMAP ? kThreads + 2 : kThreads;
-  // The tile array is padded to kP2PLdsBytes so that at most kP2PBlocksPerCu
-  // workgroups share a CU: the rest of its registers and LDS stay free for the
-  // tree-pass kernels that run beside P2P in an overlapped step (at full
-  // occupancy their small dependent launches wait for P2P workgroups to end).
+  constexpr int ROW = REMAP ? THREADS + (LOADS == 8 ? 1 : 2) : THREADS;
+  // The tile array is padded so that at most kBlocksPerCu workgroups share a
+  // CU: the rest of its registers and LDS stay free for the tree-pass kernels
+  // that run beside P2P in an overlapped step (at full occupancy their small
+  // dependent launches wait for P2P workgroups to end).
+  constexpr int kBlocksPerCu = THREADS == kThreads ? kP2PBlocksPerCu : JFM_P2P_NARROW_BLOCKS_PER_CU;
+  constexpr int kPadBytes = (kLdsPerCu / (kBlocksPerCu + 1) / 512 + 1) * 512;
   constexpr int kTileElems = LOADS * ROW;
-  constexpr int kPadElems = static_cast<int>(kP2PLdsBytes / sizeof(V4<T>));
+  constexpr int kPadElems = static_cast<int>(kPadBytes / sizeof(V4<T>));
   __shared__ V4<T> tile[kTileElems > kPadElems ? kTileElems : kPadElems];
 
   // XCD-aware order: workgroups are dealt round-robin to the XCDs, so hand
@@ -243,9 +261,9 @@ __global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos
       fetch_positions_leaf<T, WT, LOADS>(pos4, n, leaf, total, start + LOADS * tid, lst[0], next, ok);
     lst[0] = fetch_list_leaf<WT>(near_src + nb, nsl - 1, near_off, start + TILE + LOADS * tid);
   } else {
-    fetch_list<WT, FIXED, LOADS, kThreads>(leaf_size, near_src + nb, nsl - 1, near_off, start + tid, lst);
-    fetch_positions<T, WT, FIXED, LOADS, kThreads>(pos4, n, leaf_size, leaf, total, start + tid, lst, next, ok);
-    fetch_list<WT, FIXED, LOADS, kThreads>(leaf_size, near_src + nb, nsl - 1, near_off, start + TILE + tid, lst);
+    fetch_list<WT, FIXED, LOADS, THREADS>(leaf_size, near_src + nb, nsl - 1, near_off, start + tid, lst);
+    fetch_positions<T, WT, FIXED, LOADS, THREADS>(pos4, n, leaf_size, leaf, total, start + tid, lst, next, ok);
+    fetch_list<WT, FIXED, LOADS, THREADS>(leaf_size, near_src + nb, nsl - 1, near_off, start + TILE + tid, lst);
   }
   const int rbase = REMAP ? (part % LOADS) * ROW + part / LOADS : part;
   constexpr int RSTEP = REMAP ? NPART / LOADS : NPART;
@@ -271,10 +289,10 @@ __global__ __launch_bounds__(kThreads) void p2p_kernel(const T* __restrict__ pos
       fetch_positions_leaf<T, WT, LOADS>(pos4, n, leaf, total, base + TILE + LOADS * tid, lst[0], next, ok);
       lst[0] = fetch_list_leaf<WT>(near_src + nb, nsl - 1, near_off, base + 2 * TILE + LOADS * tid);
     } else {
-      fetch_positions<T, WT, FIXED, LOADS, kThreads>(pos4, n, leaf_size, leaf, total, base + TILE + tid, lst, next,
-                                                     ok);
-      fetch_list<WT, FIXED, LOADS, kThreads>(leaf_size, near_src + nb, nsl - 1, near_off, base + 2 * TILE + tid,
-                                             lst);
+      fetch_positions<T, WT, FIXED, LOADS, THREADS>(pos4, n, leaf_size, leaf, total, base + TILE + tid, lst, next,
+                                                    ok);
+      fetch_list<WT, FIXED, LOADS, THREADS>(leaf_size, near_src + nb, nsl - 1, near_off, base + 2 * TILE + tid,
+                                            lst);
     }
     // Slots past `total` hold zero-mass sources, so every partition runs the
     // same (wave-uniform, scalar-controlled) trip count.
@@ -407,12 +425,14 @@ int32_t launch_p2p_wt(hipStream_t s, const T* pos4, int32_t n, int32_t leaf_size
   const bool safe = !(eps2 > T(0));
   // the common cases (softened, compensated, leaf_size == WT) get the compile-time
   // slot decode and leaf-granular fetch; every other combination the general one
+  // float32 leaves of 8 on the fast path run in narrow workgroups
+  constexpr int kFastThreads = (sizeof(T) == 4 && WT == 8) ? kNarrowThreads : kThreads;
 #define JFM_P2P_LAUNCH(POT, COMP, SAFE)                                                                      \
   if (COMP && !SAFE && leaf_size == WT && (sizeof(T) == 8 || n <= kBufMaxRows))                              \
-    p2p_kernel<T, WT, POT, COMP, SAFE, true><<<num_blocks, kThreads, 0, s>>>(                                 \
+    p2p_kernel<T, WT, POT, COMP, SAFE, true, kFastThreads><<<num_blocks, kFastThreads, 0, s>>>(              \
         pos4, n, leaf_size, near_off, near_src, eps2, G, out_index, acc, pot, accumulate ? 1 : 0, items);    \
   else                                                                                                       \
-    p2p_kernel<T, WT, POT, COMP, SAFE, false><<<num_blocks, kThreads, 0, s>>>(                                \
+    p2p_kernel<T, WT, POT, COMP, SAFE, false, kThreads><<<num_blocks, kThreads, 0, s>>>(                      \
         pos4, n, leaf_size, near_off, near_src, eps2, G, out_index, acc, pot, accumulate ? 1 : 0, items)
   if (want_pot) {
     if (comp) {

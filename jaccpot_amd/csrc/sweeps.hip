@@ -180,6 +180,33 @@ __global__ __launch_bounds__(kWave) void p2m_group_kernel(const T* __restrict__ 
 
 // One thread per internal node of a level: centre of mass of the two children,
 // M = m2m(M_left, c_left - c) + m2m(M_right, c_right - c).
+// L2L into one node: L += l2l(L_parent, c_parent - c).
+template <typename T, int P>
+__device__ __forceinline__ void l2l_node(int node, const int32_t* __restrict__ parent,
+                                         const T* __restrict__ centers4, T* __restrict__ locals, int32_t cs) {
+  constexpr int C = ncoef(P);
+  const int par = parent[node];
+  const V4<T> cp = load4(centers4, par);
+  const V4<T> cc = load4(centers4, node);
+  T v[C];
+  load_row<T, C>(locals + static_cast<int64_t>(par) * cs, v);
+  Align<T, P> a;
+  a.init(cp.x - cc.x, cp.y - cc.y, cp.z - cc.z);  // source(parent) - destination(child)
+  rotate_all<T, P, Rot::ToZLocal>(v, a);
+  zshift_l2l<T, P>(v, a.r);
+  rotate_all<T, P, Rot::FromZLocal>(v, a);
+  T w[C];
+  load_row<T, C>(locals + static_cast<int64_t>(node) * cs, w);
+#pragma unroll
+  for (int j = 0; j < C; ++j) w[j] += v[j];
+  store_row<T, C>(locals + static_cast<int64_t>(node) * cs, w);
+}
+
+// Two lanes per internal node of a level (children already done): lane 2i
+// translates the left child's multipole to the node's centre of mass, lane
+// 2i+1 the right one's; the sum is formed as left + right and the even lane
+// writes the node.  Halves the dependent instruction chain of a level, which
+// is what a level of the upward pass waits for.
 template <typename T, int P>
 __global__ __launch_bounds__(kWave) void m2m_kernel(const int32_t* __restrict__ nodes, int32_t count,
                                                     const int32_t* __restrict__ left,
@@ -188,8 +215,11 @@ __global__ __launch_bounds__(kWave) void m2m_kernel(const int32_t* __restrict__ 
                                                     int32_t cs, T* __restrict__ centers4) {
   constexpr int C = ncoef(P);
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= count) return;
-  const int node = nodes[t];
+  const int side = t & 1;
+  // lanes past the level repeat its last node (results dropped) so that both
+  // lanes of every pair reach the exchange below
+  const bool live = (t >> 1) < count;
+  const int node = nodes[live ? (t >> 1) : count - 1];
   const int lc = left[node], rc = right[node];
   const V4<T> cl = load4(centers4, lc);
   const V4<T> cr = load4(centers4, rc);
@@ -206,25 +236,21 @@ __global__ __launch_bounds__(kWave) void m2m_kernel(const int32_t* __restrict__ 
     cy = (nl * cl.y + nr * cr.y) / (nl + nr);
     cz = (nl * cl.z + nr * cr.z) / (nl + nr);
   }
-  T acc[C];
+  const int ch = side == 0 ? lc : rc;
+  const V4<T> cc = side == 0 ? cl : cr;
+  T v[C];
+  load_row<T, C>(mult + static_cast<int64_t>(ch) * cs, v);
+  Align<T, P> a;
+  a.init(cc.x - cx, cc.y - cy, cc.z - cz);  // source - destination
+  rotate_all<T, P, Rot::ToZMultipole>(v, a);
+  zshift_m2m<T, P>(v, a.r);
+  rotate_all<T, P, Rot::FromZMultipole>(v, a);
 #pragma unroll
-  for (int j = 0; j < C; ++j) acc[j] = T(0);
-#pragma unroll
-  for (int side = 0; side < 2; ++side) {
-    const int ch = side == 0 ? lc : rc;
-    const V4<T> cc = side == 0 ? cl : cr;
-    T v[C];
-    load_row<T, C>(mult + static_cast<int64_t>(ch) * cs, v);
-    Align<T, P> a;
-    a.init(cc.x - cx, cc.y - cy, cc.z - cz);  // source - destination
-    rotate_all<T, P, Rot::ToZMultipole>(v, a);
-    zshift_m2m<T, P>(v, a.r);
-    rotate_all<T, P, Rot::FromZMultipole>(v, a);
-#pragma unroll
-    for (int j = 0; j < C; ++j) acc[j] += v[j];
+  for (int j = 0; j < C; ++j) v[j] = v[j] + __shfl_xor(v[j], 1, kWave);  // left + right on both lanes
+  if (side == 0 && live) {
+    store_row<T, C>(mult + static_cast<int64_t>(node) * cs, v);
+    store4(centers4, node, V4<T>{cx, cy, cz, tot});
   }
-  store_row<T, C>(mult + static_cast<int64_t>(node) * cs, acc);
-  store4(centers4, node, V4<T>{cx, cy, cz, tot});
 }
 
 // One thread per node of a level: L += l2l(L_parent, c_parent - c).
@@ -233,25 +259,9 @@ __global__ __launch_bounds__(kWave) void l2l_kernel(const int32_t* __restrict__ 
                                                     const int32_t* __restrict__ parent,
                                                     const T* __restrict__ centers4, T* __restrict__ locals,
                                                     int32_t cs) {
-  constexpr int C = ncoef(P);
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= count) return;
-  const int node = nodes[t];
-  const int par = parent[node];
-  const V4<T> cp = load4(centers4, par);
-  const V4<T> cc = load4(centers4, node);
-  T v[C];
-  load_row<T, C>(locals + static_cast<int64_t>(par) * cs, v);
-  Align<T, P> a;
-  a.init(cp.x - cc.x, cp.y - cc.y, cp.z - cc.z);  // source(parent) - destination(child)
-  rotate_all<T, P, Rot::ToZLocal>(v, a);
-  zshift_l2l<T, P>(v, a.r);
-  rotate_all<T, P, Rot::FromZLocal>(v, a);
-  T w[C];
-  load_row<T, C>(locals + static_cast<int64_t>(node) * cs, w);
-#pragma unroll
-  for (int j = 0; j < C; ++j) w[j] += v[j];
-  store_row<T, C>(locals + static_cast<int64_t>(node) * cs, w);
+  l2l_node<T, P>(nodes[t], parent, centers4, locals, cs);
 }
 
 // One thread per particle: far field from its leaf's local expansion.
@@ -511,7 +521,7 @@ int32_t jfm_m2m_level(void* stream, int32_t dtype, int32_t p, const int32_t* nod
   hipStream_t s = as_stream(stream);
   JFM_FOR_DTYPE(dtype, JFM_FOR_ORDER(p, {
     const int cs = coef_stride(P, sizeof(T));
-    m2m_kernel<T, P><<<blocks_for(count, kWave), kWave, 0, s>>>(nodes, count, left, right, node_range,
+    m2m_kernel<T, P><<<blocks_for(2 * static_cast<int64_t>(count), kWave), kWave, 0, s>>>(nodes, count, left, right, node_range,
                                                                 (T*)multipoles, cs, (T*)centers4);
   }));
   return check_launch("jfm_m2m_level");
