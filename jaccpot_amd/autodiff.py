@@ -19,8 +19,8 @@ from . import _native as nat
 def _check_state(state):
     if state.extra.get("gears") is not None or state.extra.get("mixed") is not None:
         raise NotImplementedError("gradients are implemented for the fixed-order, single-precision-mode path")
-    if state.order > 6:
-        raise NotImplementedError("adjoint kernels exist for orders 1..6")
+    if state.order > 7:
+        raise NotImplementedError("adjoint kernels exist for orders 1..7")
 
 
 def _recompute(fmm, state, positions: torch.Tensor, masses: torch.Tensor):

@@ -41,6 +41,7 @@ CASES = [
     ("plummer1200_p4", "plummer", 1200, 16, 4, 0.6, 1e-2),
     ("uniform2000_p3_eps0", "uniform", 2000, 8, 3, 0.5, 0.0),
     ("blobs1500_p6", "blobs", 1500, 16, 6, 0.7, 1e-2),
+    ("plummer1500_p7", "plummer", 1500, 16, 7, 0.75, 1e-2),
     # equal masses on a jittered-free lattice: many node centres share x and y
     # (far-pair separations along the z axis, where the M2L alignment angle is
     # fixed by convention and has no derivative)
