@@ -391,6 +391,18 @@ int32_t jfm_center_adjoint(void* stream, int32_t dtype, int32_t p, const int32_t
                            const int32_t* far_off, const int32_t* far_src, const void* multipoles,
                            const void* locals_own, const void* lbar_flip, const void* mbar_own_flip,
                            const void* centers4, void* cbar4);
+/* jfm_center_adjoint for per-pair orders (learned gears): pair k = (u <- v) of
+ * the far CSR ran at order gear_fwd[k], its mirror (v <- u) at gear_rev[k]
+ * (orders <= p).  With it, Mbar_own of each gear g is jfm_m2l_gear at gear g
+ * over the transposed gear-g pair lists.  Counterpart of the reference's
+ * gradient through the tagged per-gear M2L (runtime/_adaptive_policy.py
+ * :2216-2385 under runtime/fmm_evaluate.py:733), which JAX differentiates
+ * with the tags held fixed. */
+int32_t jfm_center_adjoint_gears(void* stream, int32_t dtype, int32_t p, const int32_t* nodes, int32_t num_nodes,
+                                 const int32_t* far_off, const int32_t* far_src, const int8_t* gear_fwd,
+                                 const int8_t* gear_rev, const void* multipoles, const void* locals_own,
+                                 const void* lbar_flip, const void* mbar_own_flip, const void* centers4,
+                                 void* cbar4);
 int32_t jfm_com_chain(void* stream, int32_t dtype, int32_t num_leaves, int32_t num_internal, const int32_t* parent,
                       const int32_t* node_range, const void* centers4, const void* cbar4, void* chain);
 int32_t jfm_leaf_adjoint_b(void* stream, int32_t dtype, int32_t p, const void* pos4, int32_t n, int32_t leaf_size,

@@ -7,24 +7,117 @@ to positions and masses come from hand-written adjoint kernels (P2P, L2P, M2L
 and the tree passes), recomputing every expansion in the backward pass
 instead of storing it.  ``torch.autograd.Function`` plays the role the
 reference gives ``jax.custom_vjp``.
+
+Learned gears: every far pair's M2L and centre terms are differentiated at
+the pair's own gear, with the gears held fixed (the network's choice is
+piecewise constant in the inputs, as the reference's traced gear tags are).
+Mixed precision: the recomputed expansions use the forward's float64 top
+levels; the transposed tree passes run in the working dtype.
 """
 
 from __future__ import annotations
 
+from dataclasses import dataclass
+
+import numpy as np
 import torch
 
 from . import _native as nat
 
 
 def _check_state(state):
-    if state.extra.get("gears") is not None or state.extra.get("mixed") is not None:
-        raise NotImplementedError("gradients are implemented for the fixed-order, single-precision-mode path")
     if state.order > 7:
-        raise NotImplementedError("adjoint kernels exist for orders 1..7")
+        raise NotImplementedError("adjoint kernels exist for orders 1..7 (learned gears up to 7)")
+
+
+@dataclass
+class AdjointGears:
+    """Per-pair orders of a learned-gear state, arranged for the reverse pass.
+
+    ``fwd[k]`` is the gear of far pair k = (u <- v) in CSR order, ``rev[k]``
+    the gear of its mirror (v <- u).  ``targets/num_targets/offsets/sources``
+    are, per gear g, the transposed CSR of the gear-g pairs: row v lists every
+    u whose pair (u <- v) runs at g, so that Mbar_own_v = sum over those u of
+    m2l_g(Lbar_u) (m2l_g is symmetric in the packed real basis)."""
+
+    source: object
+    gears: tuple
+    fwd: torch.Tensor
+    rev: torch.Tensor
+    targets: list
+    num_targets: list
+    offsets: list
+    sources: list
+
+
+def adjoint_gears(state) -> AdjointGears:
+    """Builds (once per gear assignment) the reverse-pass gear arrays on the device.
+
+    Two radix sorts with the pair index as payload -- of the (u, v) keys and of
+    the mirrored (v, u) keys -- put every pair next to its mirror: the far
+    lists are symmetric, so both sorted key sequences are the same and
+    position t pairs forward pair pf[t] with mirrored pair pr[t]."""
+    from .dtopology import _csr, _key_bits, _sort_u64
+    from .flatlist import pair_targets
+
+    gop = state.extra["gear_of_pair"]
+    cached = state.extra.get("adjoint_gears")
+    if cached is not None and cached.source is gop:
+        return cached
+    topo = state.topology
+    dev = state.device
+    s = nat.current_stream()
+    K = topo.num_far_pairs
+    nodes = topo.num_nodes
+    gears = tuple(state.extra["gears"].gears)
+    fwd = torch.as_tensor(np.asarray(gop)).to(device=dev, dtype=torch.int8).contiguous()
+    if fwd.numel() != K:
+        raise ValueError(f"gear_of_pair has {fwd.numel()} entries for {K} far pairs")
+    rev = torch.empty_like(fwd)
+    targets, num_targets, offsets, sources = [], [], [], []
+    one = torch.zeros(1, dtype=torch.int32, device=dev)
+    if K:
+        pt = pair_targets(state)
+        ps = state.far_src[:K]
+        kf = torch.empty(K, dtype=torch.int64, device=dev)
+        kr = torch.empty(K, dtype=torch.int64, device=dev)
+        nat.call("jfm_pair_keys", s, K, pt.data_ptr(), ps.data_ptr(), kf.data_ptr())
+        nat.call("jfm_pair_keys", s, K, ps.data_ptr(), pt.data_ptr(), kr.data_ptr())
+        idx = torch.arange(K, dtype=torch.int32, device=dev)
+        end = 32 + _key_bits(max(nodes - 1, 1))
+        sf, pf = _sort_u64(kf, end, idx)
+        sr, pr = _sort_u64(kr, end, idx)
+        del kf, kr, idx
+        if not torch.equal(sf, sr):
+            raise ValueError("the far pair lists are not symmetric; the M2L adjoint needs every mirror pair")
+        del sf
+        rev[pr.long()] = fwd[pf.long()]
+        gear_t = fwd[pr.long()]  # gear of the forward pair behind each mirrored key
+        del pf, pr
+    for g in gears:
+        if not K:
+            targets.append(one), num_targets.append(0), offsets.append(None), sources.append(one)
+            continue
+        kg = sr[gear_t == g].contiguous()
+        off, src = _csr(kg, nodes)
+        rows = torch.empty(max(nodes, 1), dtype=torch.int64, device=dev)
+        cnt = torch.zeros(1, dtype=torch.int64, device=dev)
+        nat.call("jfm_nonempty_rows", s, nodes, off.data_ptr(), rows.data_ptr(), cnt.data_ptr())
+        nt = int(cnt.item())
+        tl, _ = _sort_u64(rows[:nt], 32)
+        targets.append(tl.to(torch.int32) if nt else one)
+        num_targets.append(nt)
+        offsets.append(off)
+        sources.append(src if kg.numel() else one)
+    out = AdjointGears(gop, gears, fwd, rev, targets, num_targets, offsets, sources)
+    state.extra["adjoint_gears"] = out
+    return out
 
 
 def _recompute(fmm, state, positions: torch.Tensor, masses: torch.Tensor):
-    """Forward expansions at the live inputs; returns (L_own, L_full)."""
+    """Forward expansions at the live inputs with the forward's own passes
+    (learned gears, float64 top levels); returns the locals after M2L only
+    (the full locals are left in ``state.locals``)."""
     topo = state.topology
     s = nat.current_stream()
     code = nat.dtype_code(state.dtype)
@@ -32,18 +125,9 @@ def _recompute(fmm, state, positions: torch.Tensor, masses: torch.Tensor):
              masses.data_ptr(), state.perm.data_ptr(), topo.num_particles, state.pos4.data_ptr())
     fmm._upward(state, s, code)
     state.locals.zero_()
-    if topo.m2l_targets.size:
-        nat.call("jfm_m2l", s, code, state.order, state.m2l_targets.data_ptr(), int(topo.m2l_targets.size),
-                 state.far_off.data_ptr(), state.far_src.data_ptr(), state.multipoles.data_ptr(),
-                 state.centers4.data_ptr(), state.locals.data_ptr())
+    fmm._m2l(state, s, code)
     lown = state.locals.clone()
-    off = topo.l2l_off
-    base = state.l2l_nodes.data_ptr()
-    for lev in range(len(off) - 1):
-        a, b = int(off[lev]), int(off[lev + 1])
-        if b > a:
-            nat.call("jfm_l2l_level", s, code, state.order, base + 4 * a, b - a, state.parent.data_ptr(),
-                     state.centers4.data_ptr(), state.locals.data_ptr())
+    fmm._l2l(state, s, code)
     return lown
 
 
@@ -58,6 +142,7 @@ def fmm_vjp(fmm, state, positions: torch.Tensor, masses: torch.Tensor, acc_bar: 
     s = nat.current_stream()
     x = positions.to(dev, dt).contiguous()
     m = masses.to(dev, dt).contiguous()
+    ag = adjoint_gears(state) if state.extra.get("gears") is not None else None
     lown = _recompute(fmm, state, x, m)
     n = topo.num_particles
     cot4 = torch.empty((n, 4), dtype=dt, device=dev)
@@ -95,11 +180,22 @@ def fmm_vjp(fmm, state, positions: torch.Tensor, masses: torch.Tensor, acc_bar: 
     cbar4 = torch.zeros((topo.num_nodes, 4), dtype=dt, device=dev)
     if topo.m2l_targets.size:
         nt = int(topo.m2l_targets.size)
-        nat.call("jfm_m2l", s, code, state.order, state.m2l_targets.data_ptr(), nt, state.far_off.data_ptr(),
-                 state.far_src.data_ptr(), lbar.data_ptr(), state.centers4.data_ptr(), mbar.data_ptr())
-        nat.call("jfm_center_adjoint", s, code, state.order, state.m2l_targets.data_ptr(), nt,
-                 state.far_off.data_ptr(), state.far_src.data_ptr(), state.multipoles.data_ptr(), lown.data_ptr(),
-                 lbar.data_ptr(), mbar.data_ptr(), state.centers4.data_ptr(), cbar4.data_ptr())
+        if ag is None:
+            nat.call("jfm_m2l", s, code, state.order, state.m2l_targets.data_ptr(), nt, state.far_off.data_ptr(),
+                     state.far_src.data_ptr(), lbar.data_ptr(), state.centers4.data_ptr(), mbar.data_ptr())
+            nat.call("jfm_center_adjoint", s, code, state.order, state.m2l_targets.data_ptr(), nt,
+                     state.far_off.data_ptr(), state.far_src.data_ptr(), state.multipoles.data_ptr(),
+                     lown.data_ptr(), lbar.data_ptr(), mbar.data_ptr(), state.centers4.data_ptr(), cbar4.data_ptr())
+        else:
+            for i, g in enumerate(ag.gears):
+                if ag.num_targets[i]:
+                    nat.call("jfm_m2l_gear", s, code, state.order, g, 1, ag.targets[i].data_ptr(),
+                             ag.num_targets[i], ag.offsets[i].data_ptr(), ag.sources[i].data_ptr(), lbar.data_ptr(),
+                             state.centers4.data_ptr(), mbar.data_ptr())
+            nat.call("jfm_center_adjoint_gears", s, code, state.order, state.m2l_targets.data_ptr(), nt,
+                     state.far_off.data_ptr(), state.far_src.data_ptr(), ag.fwd.data_ptr(), ag.rev.data_ptr(),
+                     state.multipoles.data_ptr(), lown.data_ptr(), lbar.data_ptr(), mbar.data_ptr(),
+                     state.centers4.data_ptr(), cbar4.data_ptr())
     # cotangents of the multipoles down the tree (m2m^T = l2l on parity-flipped rows)
     off = topo.l2l_off
     for lev in range(len(off) - 1):

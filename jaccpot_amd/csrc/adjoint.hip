@@ -362,15 +362,36 @@ __device__ __forceinline__ void m2l_bilinear_grad(const T* A, const T* B, T dx, 
   gz = -sz;
 }
 
+// m2l_bilinear_grad at a run-time order g in [G, P] (rows hold at least
+// ncoef(g) coefficients; only that prefix is read).
+template <typename T, int G, int P>
+__device__ __forceinline__ void m2l_bilinear_grad_at(int g, const T* A, const T* B, T dx, T dy, T dz, T& gx, T& gy,
+                                                     T& gz) {
+  if constexpr (G < P) {
+    if (g <= G) {
+      m2l_bilinear_grad<T, G>(A, B, dx, dy, dz, gx, gy, gz);
+      return;
+    }
+    m2l_bilinear_grad_at<T, G + 1, P>(g, A, B, dx, dy, dz, gx, gy, gz);
+  } else {
+    m2l_bilinear_grad<T, P>(A, B, dx, dy, dz, gx, gy, gz);
+  }
+}
+
 // One wave per node u with far partners: lanes over partners v, summing
 //   + d/dd [Lbar_u . m2l(M_v, d)] at d = c_u - c_v      (pair u <- v)
 //   - d/dd [Lbar_v . m2l(M_u, d)] at d = c_v - c_u      (pair v <- u)
 // and lane 0 adds Gr(Lown_u) . Lbar_u - Gr(Mbar_own_u) . M_u.  Writes cbar4[u] (xyz).
-template <typename T, int P>
+// GEARS: pair k = (u <- v) of the CSR ran at order gear_fwd[k] and its mirror
+// (v <- u) at gear_rev[k]; each bilinear term runs at its pair's order (the
+// order-g operator keeps the terms n + k <= g, so it is not the order-P one
+// with rows cut at degree g).
+template <typename T, int P, bool GEARS>
 __global__ __launch_bounds__(kWave) void center_adjoint_kernel(
     const int32_t* __restrict__ nodes, const int32_t* __restrict__ far_off, const int32_t* __restrict__ far_src,
-    const T* __restrict__ mult, const T* __restrict__ lown, const T* __restrict__ lbar_flip,
-    const T* __restrict__ mbar_own_flip, int32_t cs, const T* __restrict__ centers4, T* __restrict__ cbar4) {
+    const int8_t* __restrict__ gear_fwd, const int8_t* __restrict__ gear_rev, const T* __restrict__ mult,
+    const T* __restrict__ lown, const T* __restrict__ lbar_flip, const T* __restrict__ mbar_own_flip, int32_t cs,
+    const T* __restrict__ centers4, T* __restrict__ cbar4) {
   constexpr int C = ncoef(P);
   __shared__ T red[kWave][4];
   const int u = nodes[blockIdx.x];
@@ -399,8 +420,13 @@ __global__ __launch_bounds__(kWave) void center_adjoint_kernel(
     }
     const T dx = cu.x - cv.x, dy = cu.y - cv.y, dz = cu.z - cv.z;
     T ax, ay, az, bx, by, bz;
-    m2l_bilinear_grad<T, P>(Lu, Mv, dx, dy, dz, ax, ay, az);
-    m2l_bilinear_grad<T, P>(Lv, Mu, -dx, -dy, -dz, bx, by, bz);
+    if constexpr (GEARS) {
+      m2l_bilinear_grad_at<T, 1, P>(gear_fwd[k], Lu, Mv, dx, dy, dz, ax, ay, az);
+      m2l_bilinear_grad_at<T, 1, P>(gear_rev[k], Lv, Mu, -dx, -dy, -dz, bx, by, bz);
+    } else {
+      m2l_bilinear_grad<T, P>(Lu, Mv, dx, dy, dz, ax, ay, az);
+      m2l_bilinear_grad<T, P>(Lv, Mu, -dx, -dy, -dz, bx, by, bz);
+    }
     gx += ax - bx;
     gy += ay - by;
     gz += az - bz;
@@ -640,12 +666,30 @@ int32_t jfm_center_adjoint(void* stream, int32_t dtype, int32_t p, const int32_t
     return fail(JFM_ERR_ARG, "jfm_center_adjoint: NULL buffer");
   hipStream_t s = as_stream(stream);
   JFM_FOR_DTYPE(dtype, JFM_FOR_ADJ_ORDER(p, {
-    center_adjoint_kernel<T, P><<<num_nodes, kWave, 0, s>>>(nodes, far_off, far_src, (const T*)multipoles,
-                                                            (const T*)locals_own, (const T*)lbar_flip,
-                                                            (const T*)mbar_own_flip, coef_stride(P, sizeof(T)),
-                                                            (const T*)centers4, (T*)cbar4);
+    center_adjoint_kernel<T, P, false><<<num_nodes, kWave, 0, s>>>(
+        nodes, far_off, far_src, nullptr, nullptr, (const T*)multipoles, (const T*)locals_own,
+        (const T*)lbar_flip, (const T*)mbar_own_flip, coef_stride(P, sizeof(T)), (const T*)centers4, (T*)cbar4);
   }));
   return check_launch("jfm_center_adjoint");
+}
+
+int32_t jfm_center_adjoint_gears(void* stream, int32_t dtype, int32_t p, const int32_t* nodes, int32_t num_nodes,
+                                 const int32_t* far_off, const int32_t* far_src, const int8_t* gear_fwd,
+                                 const int8_t* gear_rev, const void* multipoles, const void* locals_own,
+                                 const void* lbar_flip, const void* mbar_own_flip, const void* centers4,
+                                 void* cbar4) {
+  if (num_nodes < 0) return fail(JFM_ERR_ARG, "jfm_center_adjoint_gears: bad sizes");
+  if (num_nodes == 0) return JFM_OK;
+  if (!nodes || !far_off || !far_src || !gear_fwd || !gear_rev || !multipoles || !locals_own || !lbar_flip ||
+      !mbar_own_flip || !centers4 || !cbar4)
+    return fail(JFM_ERR_ARG, "jfm_center_adjoint_gears: NULL buffer");
+  hipStream_t s = as_stream(stream);
+  JFM_FOR_DTYPE(dtype, JFM_FOR_ADJ_ORDER(p, {
+    center_adjoint_kernel<T, P, true><<<num_nodes, kWave, 0, s>>>(
+        nodes, far_off, far_src, gear_fwd, gear_rev, (const T*)multipoles, (const T*)locals_own,
+        (const T*)lbar_flip, (const T*)mbar_own_flip, coef_stride(P, sizeof(T)), (const T*)centers4, (T*)cbar4);
+  }));
+  return check_launch("jfm_center_adjoint_gears");
 }
 
 int32_t jfm_com_chain(void* stream, int32_t dtype, int32_t num_leaves, int32_t num_internal, const int32_t* parent,

@@ -520,9 +520,14 @@ class FastMultipoleMethod:
                      state.centers4.data_ptr(), mixed["mult64"].data_ptr(), mixed["centers64"].data_ptr())
 
     def _downward(self, state: FMMPreparedState, s, code) -> None:
-        topo = state.topology
         state.locals.zero_()
-        # library-wide engine of the float32 M2L rotations, set before every downward pass
+        self._m2l(state, s, code)
+        self._l2l(state, s, code)
+
+    def _m2l(self, state: FMMPreparedState, s, code) -> None:
+        """Adds every far pair's M2L (per gear with learned orders) into the locals."""
+        topo = state.topology
+        # library-wide engine of the float32 M2L rotations, set before every M2L pass
         nat.call("jfm_set_m2l_engine", nat.M2L_ENGINES[self.m2l_engine])
         sched = state.extra.get("gears")
         if sched is not None:
@@ -539,6 +544,11 @@ class FastMultipoleMethod:
             nat.call("jfm_m2l", s, code, state.order, state.m2l_targets.data_ptr(), int(topo.m2l_targets.size),
                      state.far_off.data_ptr(), state.far_src.data_ptr(), state.multipoles.data_ptr(),
                      state.centers4.data_ptr(), state.locals.data_ptr())
+
+    @staticmethod
+    def _l2l(state: FMMPreparedState, s, code) -> None:
+        """Pushes the locals down the tree (float64 top levels in mixed precision)."""
+        topo = state.topology
         mixed = state.extra.get("mixed")
         first_lev = 0
         if mixed is not None and mixed["ntop"]:

@@ -171,3 +171,94 @@ def test_split_near_lists_give_the_same_gradient(eps, require_gpu):
     gx1, gm1 = _loss_and_grad(fmm, state, x, m, w, u)
     assert np.linalg.norm(gx1 - gx0) / np.linalg.norm(gx0) < 1e-13
     assert np.linalg.norm(gm1 - gm0) / np.linalg.norm(gm0) < 1e-13
+
+
+def test_per_pair_gear_gradient_matches_central_differences(require_gpu):
+    """Per-pair M2L orders (gears 2, 4, 6 drawn at random per far pair, so a
+    pair and its mirror mostly run at different orders): the adjoint of the
+    gear M2L and of its centre terms against float64 central differences at
+    the frozen topology and gears."""
+    from jaccpot_amd import FastMultipoleMethod
+    from jaccpot_amd.autodiff import adjoint_gears
+    from jaccpot_amd.learned import build_gear_schedule
+
+    n = 1500
+    print("[gpu] adjoint fd, random per-pair gears (2,4,6), plummer 1500", flush=True)
+    x, m = ics.plummer(n, 44)
+    rng = np.random.default_rng(10)
+    w = rng.normal(size=(n, 3))
+    u = rng.normal(size=n)
+    fmm = FastMultipoleMethod(precision="fp64", theta=0.7, softening=1e-2)
+    state = fmm.prepare_state(x, m, leaf_size=16, max_order=6)
+    K = state.topology.num_far_pairs
+    g = rng.choice(np.array([2, 4, 6], np.int8), size=K)
+    state.extra["gears"] = build_gear_schedule(state, g, (2, 4, 6))
+    state.extra["gear_of_pair"] = g
+    ag = adjoint_gears(state)
+    fwd, rev = ag.fwd.cpu().numpy(), ag.rev.cpu().numpy()
+    assert np.array_equal(fwd, g) and (fwd != rev).mean() > 0.5
+    # the mirror of the mirror is the pair itself
+    assert np.bincount(rev, minlength=7)[[2, 4, 6]].tolist() == np.bincount(g, minlength=7)[[2, 4, 6]].tolist()
+    gx, gm = _loss_and_grad(fmm, state, x, m, w, u)
+    fds, ans = [], []
+    for k in range(6):
+        vx = rng.normal(size=x.shape)
+        vm = rng.normal(size=m.shape) * np.abs(m).mean()
+        h = 1e-6
+        fd = (_loss(fmm, state, x + h * vx, m + h * vm, w, u) - _loss(fmm, state, x - h * vx, m - h * vm, w, u)) / (
+            2 * h)
+        an = float((gx * vx).sum() + (gm * vm).sum())
+        print(f"[gpu]   dir {k}: fd {fd:.10e} adjoint {an:.10e} rel {abs(fd - an) / max(abs(an), 1e-30):.2e}",
+              flush=True)
+        fds.append(fd)
+        ans.append(an)
+    rms = float(np.sqrt(np.mean(np.square(ans))))
+    for fd, an in zip(fds, ans):
+        assert abs(fd - an) <= 1e-6 * max(abs(an), rms)
+    # the gears change the gradient: the all-top-gear gradient is a different one
+    state.extra.pop("gears")
+    state.extra.pop("gear_of_pair")
+    gx6, _ = _loss_and_grad(fmm, state, x, m, w, u)
+    assert np.linalg.norm(gx6 - gx) > 1e-8 * np.linalg.norm(gx)
+
+
+def test_learned_mixed_fp32_gradient_matches_fp64(require_gpu):
+    """The learned (2, 4, 6) model with float64 top levels in float32 on a
+    composite galaxy: autograd gradients of the C5-style loss against the
+    float64 gradient at the same topology and the same per-pair gears."""
+    from jaccpot_amd import FastMultipoleMethod
+    from jaccpot_amd.learned import build_gear_schedule
+
+    n = 16384
+    print("[gpu] learned+mixed fp32 gradient vs fp64, galaxy 16384 gears (2,4,6)", flush=True)
+    x, m = ics.galaxy(n, 5)
+    x32, m32 = x.astype(np.float32), m.astype(np.float32)
+    fmm32 = FastMultipoleMethod(precision="fp32", theta=0.6, softening=1e-3, adaptive_order=True, p_gears=(2, 4, 6),
+                                gear_tolerance=-1.5, mixed_precision=True, order_judge_targets=0)
+    st32 = fmm32.prepare_state(x32, m32, leaf_size=16, max_order=6)
+    assert st32.extra.get("mixed") is not None
+    g = st32.extra["gear_of_pair"]
+    counts = {int(k): int(np.sum(g == k)) for k in (2, 4, 6)}
+    print(f"[gpu]   gears {counts}", flush=True)
+    assert min(counts.values()) >= 0.005 * g.size
+    xt = torch.as_tensor(x32, device="cuda").requires_grad_(True)
+    mt = torch.as_tensor(m32, device="cuda").requires_grad_(True)
+    acc, pot = fmm32.differentiable_accelerations_and_potential(st32, xt, mt)
+    loss = (mt * pot)[:1024].sum() + 0.5 * (acc[:1024] * acc[:1024]).sum()
+    loss.backward()
+
+    fmm64 = FastMultipoleMethod(precision="fp64", theta=0.6, softening=1e-3)
+    st64 = fmm64.prepare_state(x32.astype(np.float64), m32.astype(np.float64), leaf_size=16, max_order=6)
+    t32, t64 = st32.topology, st64.topology
+    assert np.array_equal(t32.far_off, t64.far_off) and np.array_equal(t32.far_src, t64.far_src)
+    st64.extra["gears"] = build_gear_schedule(st64, g, (2, 4, 6))
+    st64.extra["gear_of_pair"] = g
+    x64 = torch.as_tensor(x32.astype(np.float64), device="cuda").requires_grad_(True)
+    m64 = torch.as_tensor(m32.astype(np.float64), device="cuda").requires_grad_(True)
+    a64, p64 = fmm64.differentiable_accelerations_and_potential(st64, x64, m64)
+    l64 = (m64 * p64)[:1024].sum() + 0.5 * (a64[:1024] * a64[:1024]).sum()
+    l64.backward()
+    ex = (torch.linalg.norm(xt.grad.double() - x64.grad) / torch.linalg.norm(x64.grad)).item()
+    em = (torch.linalg.norm(mt.grad.double() - m64.grad) / torch.linalg.norm(m64.grad)).item()
+    print(f"[gpu]   fp32 learned+mixed vs fp64 same gears: positions {ex:.3e} masses {em:.3e}", flush=True)
+    assert ex < 1e-4 and em < 1e-4
