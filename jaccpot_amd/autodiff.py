@@ -26,8 +26,8 @@ from . import _native as nat
 
 
 def _check_state(state):
-    if state.order > 7:
-        raise NotImplementedError("adjoint kernels exist for orders 1..7 (learned gears up to 7)")
+    if not 1 <= state.order <= 8:
+        raise NotImplementedError("adjoint kernels exist for orders 1..8")
 
 
 @dataclass

@@ -30,8 +30,8 @@
 #include "launch.hpp"
 #include "rows.hpp"
 
-// Orders with adjoint kernels (p <= 7; the centre pass translates at order
-// p + 1, and the B_U tables reach degree 8).
+// Orders with adjoint kernels (every order of the forward path; the centre
+// pass translates at order p + 1, for which the B_U tables reach degree 9).
 #define JFM_FOR_ADJ_ORDER(p, ...)                                                      \
   switch (p) {                                                                         \
     case 1: { constexpr int P = 1; __VA_ARGS__; } break;                               \
@@ -41,8 +41,9 @@
     case 5: { constexpr int P = 5; __VA_ARGS__; } break;                               \
     case 6: { constexpr int P = 6; __VA_ARGS__; } break;                               \
     case 7: { constexpr int P = 7; __VA_ARGS__; } break;                               \
+    case 8: { constexpr int P = 8; __VA_ARGS__; } break;                               \
     default:                                                                           \
-      return ::jfm::fail(JFM_ERR_UNSUPPORTED, "adjoint kernels exist for orders 1..7"); \
+      return ::jfm::fail(JFM_ERR_UNSUPPORTED, "adjoint kernels exist for orders 1..8"); \
   }
 
 namespace jfm {

@@ -132,9 +132,11 @@ int main(int argc, char** argv) {
   FILE* f = std::fopen(argv[1], "w");
   if (!f) return 1;
   std::fprintf(f, "// generated by gen_tables.cpp -- do not edit\n#pragma once\n\nnamespace jfm {\n\n");
-  std::fprintf(f, "// B_U blocks for degrees 0..%d, row-major, block l at offset l(2l-1)(2l+1)/3\n", JFM_MAX_ORDER);
+  // one degree past JFM_MAX_ORDER: the centre adjoint translates at order p + 1
+  std::fprintf(f, "// B_U blocks for degrees 0..%d, row-major, block l at offset l(2l-1)(2l+1)/3\n",
+               JFM_MAX_ORDER + 1);
   std::fprintf(f, "constexpr double kBU[] = {\n");
-  for (int ell = 0; ell <= JFM_MAX_ORDER; ++ell) {
+  for (int ell = 0; ell <= JFM_MAX_ORDER + 1; ++ell) {
     std::vector<double> bu = jfm::dehnen_bu(ell);
     std::fprintf(f, "  // degree %d\n", ell);
     for (size_t k = 0; k < bu.size(); ++k) {

@@ -42,6 +42,7 @@ CASES = [
     ("uniform2000_p3_eps0", "uniform", 2000, 8, 3, 0.5, 0.0),
     ("blobs1500_p6", "blobs", 1500, 16, 6, 0.7, 1e-2),
     ("plummer1500_p7", "plummer", 1500, 16, 7, 0.75, 1e-2),
+    ("plummer1500_p8", "plummer", 1500, 16, 8, 0.75, 1e-2),
     # equal masses on a jittered-free lattice: many node centres share x and y
     # (far-pair separations along the z axis, where the M2L alignment angle is
     # fixed by convention and has no derivative)
@@ -222,23 +223,32 @@ def test_per_pair_gear_gradient_matches_central_differences(require_gpu):
     assert np.linalg.norm(gx6 - gx) > 1e-8 * np.linalg.norm(gx)
 
 
-def test_learned_mixed_fp32_gradient_matches_fp64(require_gpu):
-    """The learned (2, 4, 6) model with float64 top levels in float32 on a
-    composite galaxy: autograd gradients of the C5-style loss against the
-    float64 gradient at the same topology and the same per-pair gears."""
+LEARNED_GRAD_CASES = [
+    # name, generator, gears, leaf
+    ("galaxy16384_gears246", "galaxy", (2, 4, 6), 16),
+    ("plummer16384_gears368", "plummer", (3, 6, 8), 8),
+]
+
+
+@pytest.mark.parametrize("case", LEARNED_GRAD_CASES, ids=[c[0] for c in LEARNED_GRAD_CASES])
+def test_learned_mixed_fp32_gradient_matches_fp64(case, require_gpu):
+    """A shipped learned model with float64 top levels in float32: autograd
+    gradients of the C5-style loss against the float64 gradient at the same
+    topology and the same per-pair gears."""
     from jaccpot_amd import FastMultipoleMethod
     from jaccpot_amd.learned import build_gear_schedule
 
+    name, gen, gears, leaf = case
     n = 16384
-    print("[gpu] learned+mixed fp32 gradient vs fp64, galaxy 16384 gears (2,4,6)", flush=True)
-    x, m = ics.galaxy(n, 5)
+    print(f"[gpu] learned+mixed fp32 gradient vs fp64, {name}", flush=True)
+    x, m = getattr(ics, gen)(n, 5)
     x32, m32 = x.astype(np.float32), m.astype(np.float32)
-    fmm32 = FastMultipoleMethod(precision="fp32", theta=0.6, softening=1e-3, adaptive_order=True, p_gears=(2, 4, 6),
+    fmm32 = FastMultipoleMethod(precision="fp32", theta=0.6, softening=1e-3, adaptive_order=True, p_gears=gears,
                                 gear_tolerance=-1.5, mixed_precision=True, order_judge_targets=0)
-    st32 = fmm32.prepare_state(x32, m32, leaf_size=16, max_order=6)
+    st32 = fmm32.prepare_state(x32, m32, leaf_size=leaf, max_order=gears[-1])
     assert st32.extra.get("mixed") is not None
     g = st32.extra["gear_of_pair"]
-    counts = {int(k): int(np.sum(g == k)) for k in (2, 4, 6)}
+    counts = {int(k): int(np.sum(g == k)) for k in gears}
     print(f"[gpu]   gears {counts}", flush=True)
     assert min(counts.values()) >= 0.005 * g.size
     xt = torch.as_tensor(x32, device="cuda").requires_grad_(True)
@@ -248,10 +258,10 @@ def test_learned_mixed_fp32_gradient_matches_fp64(require_gpu):
     loss.backward()
 
     fmm64 = FastMultipoleMethod(precision="fp64", theta=0.6, softening=1e-3)
-    st64 = fmm64.prepare_state(x32.astype(np.float64), m32.astype(np.float64), leaf_size=16, max_order=6)
+    st64 = fmm64.prepare_state(x32.astype(np.float64), m32.astype(np.float64), leaf_size=leaf, max_order=gears[-1])
     t32, t64 = st32.topology, st64.topology
     assert np.array_equal(t32.far_off, t64.far_off) and np.array_equal(t32.far_src, t64.far_src)
-    st64.extra["gears"] = build_gear_schedule(st64, g, (2, 4, 6))
+    st64.extra["gears"] = build_gear_schedule(st64, g, gears)
     st64.extra["gear_of_pair"] = g
     x64 = torch.as_tensor(x32.astype(np.float64), device="cuda").requires_grad_(True)
     m64 = torch.as_tensor(m32.astype(np.float64), device="cuda").requires_grad_(True)
