@@ -479,6 +479,47 @@ int32_t jfm_direct_sum_f64_static(void* stream, const double* pos4, int32_t n, c
                                   double eps2, double G, double* acc, double* pot);
 
 /* ------------------------------------------------------------------------
+ * Momentum-conserving (mutual) lane with block-step level weights, the
+ * reference's jaccpot/mutual (force.py:985-1068: mutual_weighted_accelerations;
+ * nearfield.py: mutual_near_field_forces; farfield.py:
+ * mutual_far_field_forces).  The topology's far and near lists are symmetric
+ * (every unordered pair once, both directions emitted), so each pair's two
+ * halves come from the same operator; interaction level of a particle pair
+ * is max(rung_i, rung_j), of a cell pair max(rung_A, rung_B) with the cell
+ * rung the maximum over its particles (force.py:885-960).
+ *
+ * jfm_mutual_rungs: rung_sorted[j] = rung[perm[j]] and node_rung[leaf node]
+ * = max over the leaf (-1 if empty); node_rung [nodes].
+ * jfm_mutual_rungs_level: node_rung[u] = max(node_rung[left[u]],
+ * node_rung[right[u]]) for u in nodes[count] (one tree level, deepest first). */
+int32_t jfm_mutual_rungs(void* stream, int32_t n, int32_t num_leaves, int32_t leaf_size, const int32_t* perm,
+                         const int32_t* rung, int32_t* rung_sorted, int32_t* node_rung);
+int32_t jfm_mutual_rungs_level(void* stream, const int32_t* nodes, int32_t count, const int32_t* left,
+                               const int32_t* right, int32_t* node_rung);
+/* Near-field forces F_i = sum_j G m_i m_j w[max(r_i, r_j)] (x_j - x_i) /
+ * (|x_j - x_i|^2 + eps2)^{3/2} over the self leaf (j != i) and the near
+ * leaves, with f_ij == -f_ji to the last bit; rows force[out_index[i]][3]
+ * (out_index NULL: tree order).  level_weights NULL: every pair weight 1
+ * (rung_sorted unused).  leaf_size <= 256, num_levels <= 64. */
+int32_t jfm_mutual_p2p(void* stream, int32_t dtype, const void* pos4, int32_t n, int32_t num_leaves,
+                       int32_t leaf_size, const int32_t* near_off, const int32_t* near_src, const int32_t* rung_sorted,
+                       const void* level_weights, int32_t num_levels, double eps2, double G, const int32_t* out_index,
+                       void* force);
+/* jfm_m2l_flat at gear p with pair (t <- s) scaled by
+ * level_weights[clamp(max(node_rung[t], node_rung[s]), 0, num_levels - 1)]
+ * (level_weights in device memory, the working dtype). */
+int32_t jfm_m2l_flat_weighted(void* stream, int32_t dtype, int32_t p, int32_t num_pairs, const int32_t* pair_tgt,
+                              const int32_t* pair_src, const int32_t* node_rung, const void* level_weights,
+                              int32_t num_levels, const void* multipoles, const void* centers4, void* locals,
+                              void* scratch, int32_t num_span, const int32_t* span_tgt, const int32_t* span_w0,
+                              const int32_t* span_w1);
+/* Per input row r: far_force = m_r far_acc[r], acc = (near_force + far_force)
+ * / m_r (0 where m_r <= 0); far_force may be NULL.  All [n][3] except
+ * masses [n], in input order. */
+int32_t jfm_mutual_combine(void* stream, int32_t dtype, int32_t n, const void* masses, const void* far_acc,
+                           const void* near_force, void* acc, void* far_force);
+
+/* ------------------------------------------------------------------------
  * Validation only: host (CPU) instantiation of the kernels' per-pair operator
  * math, one pair per call, for checking against the float64 oracle without a
  * GPU.  No product path uses it; it has no counterpart in the reference.

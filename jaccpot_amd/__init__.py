@@ -9,6 +9,7 @@ through the C ABI of ``libjfm.so`` (include/jfm.h).
 from .config import (FarFieldConfig, FMMAdvancedConfig, GradConfig, NearFieldConfig, RuntimePolicyConfig,
                      TreeConfig)
 from .direct import direct_sum_gravitational_acceleration
+from .mutual import BlockStepFMM
 from .solver import FastMultipoleMethod, FMMPreparedState
 from .topology import Topology, build_topology
 
@@ -19,6 +20,7 @@ __all__ = [
     "NearFieldConfig",
     "RuntimePolicyConfig",
     "TreeConfig",
+    "BlockStepFMM",
     "FastMultipoleMethod",
     "FMMPreparedState",
     "Topology",

@@ -236,6 +236,15 @@ __global__ __launch_bounds__(kWave) void m2l_kernel(const int32_t* __restrict__ 
     for (int j = C + lane; j < cs; j += kWave) row[j] = T(0);
 }
 
+// Per-pair weights of the flat M2L (jfm_m2l_flat_weighted): pair (t <- s)
+// is scaled by level_w[clamp(max(node_rung[t], node_rung[s]), 0, nlev - 1)].
+template <typename T>
+struct PairWeights {
+  const int32_t* node_rung;
+  const T* level_w;
+  int32_t nlev;
+};
+
 // Flat mapping (jfm_m2l_flat): one wave per 64 consecutive pairs of a CSR
 // far list (pairs grouped by target, pair_tgt[k] = target of pair k); lane l
 // runs pair 64 w + l.  The lanes' contributions go to LDS and lane j sums
@@ -247,12 +256,13 @@ __global__ __launch_bounds__(kWave) void m2l_kernel(const int32_t* __restrict__ 
 // and continues), and m2l_fixup_kernel adds those partials in wave order.
 // Deterministic, no atomics; every lane carries a pair whatever the number
 // of pairs per target.
-template <typename T, int P, int G, bool MF>
+// WT: every pair's contribution is scaled by its PairWeights weight.
+template <typename T, int P, int G, bool MF, bool WT = false>
 __global__ __launch_bounds__(kWave) void m2l_flat_kernel(int32_t K, const int32_t* __restrict__ pair_tgt,
                                                          const int32_t* __restrict__ pair_src,
                                                          const T* __restrict__ mult, const T* __restrict__ centers4,
                                                          T* __restrict__ locals, int32_t cs,
-                                                         T* __restrict__ scratch) {
+                                                         T* __restrict__ scratch, PairWeights<T> pw) {
   constexpr int C = ncoef(G);
   __shared__ T red[kWave][kChunk + 1];
   __shared__ int32_t tg[kWave];
@@ -276,6 +286,12 @@ __global__ __launch_bounds__(kWave) void m2l_flat_kernel(int32_t K, const int32_
     T v[C];
     load_row<T, C>(mult + static_cast<int64_t>(src) * cs, v);
     m2l_pair<T, G, MF>(v, w, ct.x - cs4.x, ct.y - cs4.y, ct.z - cs4.z, bu);  // target - source
+    if constexpr (WT) {
+      const int lv = min(max(max(pw.node_rung[tt], pw.node_rung[src]), 0), pw.nlev - 1);
+      const T f = pw.level_w[lv];
+#pragma unroll
+      for (int j = 0; j < C; ++j) w[j] *= f;
+    }
     if (live) {
       t = tt;
     } else {
@@ -462,15 +478,16 @@ template <typename T, int P, int G>
 void launch_flat_pairs(hipStream_t s, int nw, int32_t K, const int32_t* pair_tgt, const int32_t* pair_src,
                        const void* mult, const void* centers4, void* locals, void* scratch) {
   const int cs = coef_stride(P, sizeof(T));
+  const PairWeights<T> none{nullptr, nullptr, 0};
   if constexpr (std::is_same<T, float>::value) {
     if (use_mfma<T>()) {
       m2l_flat_kernel<T, P, G, true><<<nw, kWave, 0, s>>>(K, pair_tgt, pair_src, (const T*)mult,
-                                                           (const T*)centers4, (T*)locals, cs, (T*)scratch);
+                                                           (const T*)centers4, (T*)locals, cs, (T*)scratch, none);
       return;
     }
   }
   m2l_flat_kernel<T, P, G, false><<<nw, kWave, 0, s>>>(K, pair_tgt, pair_src, (const T*)mult, (const T*)centers4,
-                                                        (T*)locals, cs, (T*)scratch);
+                                                        (T*)locals, cs, (T*)scratch, none);
 }
 
 template <typename T, int P>
@@ -509,6 +526,33 @@ extern "C" int32_t jfm_m2l_flat(void* stream, int32_t dtype, int32_t p, int32_t 
   JFM_FOR_DTYPE(dtype, JFM_FOR_ORDER(p, {
     return launch_flat<T, P>(s, gear, num_pairs, pair_tgt, pair_src, multipoles, centers4, locals, scratch,
                              num_span, span_tgt, span_w0, span_w1);
+  }));
+  return JFM_OK;
+}
+
+extern "C" int32_t jfm_m2l_flat_weighted(void* stream, int32_t dtype, int32_t p, int32_t num_pairs,
+                                         const int32_t* pair_tgt, const int32_t* pair_src, const int32_t* node_rung,
+                                         const void* level_weights, int32_t num_levels, const void* multipoles,
+                                         const void* centers4, void* locals, void* scratch, int32_t num_span,
+                                         const int32_t* span_tgt, const int32_t* span_w0, const int32_t* span_w1) {
+  if (num_pairs < 0 || num_span < 0) return fail(JFM_ERR_ARG, "jfm_m2l_flat_weighted: negative count");
+  if (num_levels < 1) return fail(JFM_ERR_ARG, "jfm_m2l_flat_weighted: need at least one level weight");
+  if (num_pairs == 0) return JFM_OK;
+  if (!pair_tgt || !pair_src || !node_rung || !level_weights || !multipoles || !centers4 || !locals || !scratch)
+    return fail(JFM_ERR_ARG, "jfm_m2l_flat_weighted: NULL buffer");
+  if (num_span > 0 && (!span_tgt || !span_w0 || !span_w1))
+    return fail(JFM_ERR_ARG, "jfm_m2l_flat_weighted: NULL span list");
+  hipStream_t s = as_stream(stream);
+  const int nw = (num_pairs + kWave - 1) / kWave;
+  JFM_FOR_DTYPE(dtype, JFM_FOR_ORDER(p, {
+    const int cs = coef_stride(P, sizeof(T));
+    const PairWeights<T> pw{node_rung, (const T*)level_weights, num_levels};
+    m2l_flat_kernel<T, P, P, false, true><<<nw, kWave, 0, s>>>(num_pairs, pair_tgt, pair_src, (const T*)multipoles,
+                                                               (const T*)centers4, (T*)locals, cs, (T*)scratch, pw);
+    if (num_span > 0)
+      m2l_fixup_kernel<T, P><<<(num_span + 3) / 4, 4 * kWave, 0, s>>>(num_span, span_tgt, span_w0, span_w1,
+                                                                     (const T*)scratch, (T*)locals, cs);
+    return check_launch("jfm_m2l_flat_weighted");
   }));
   return JFM_OK;
 }
