@@ -104,10 +104,13 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="plummer4m_p6", choices=sorted(WORKLOADS))
-    ap.add_argument("--mode", default="learned", choices=("learned", "plain", "learned-only", "grad-only"),
+    ap.add_argument("--mode", default="learned",
+                    choices=("learned", "plain", "learned-only", "grad-only", "mutual"),
                     help="learned: time the learned+mixed FMM (value) and the plain one beside it; "
                          "learned-only: the learned+mixed FMM alone (profiling); "
-                         "grad-only: the differentiable forward + backward alone (profiling; prints its object)")
+                         "grad-only: the differentiable forward + backward alone (profiling; prints its object); "
+                         "mutual: the momentum-conserving block-step lane (prints its own line)")
+    ap.add_argument("--mutual-leaf", type=int, default=8, help="leaf size of --mode mutual")
     ap.add_argument("--graph", action="store_true",
                     help="also time the step replayed from a captured HIP graph and report that as ms_per_step")
     ap.add_argument("--no-grad", action="store_true", help="skip the differentiable-path timing")
@@ -138,6 +141,9 @@ def main() -> int:
 
     if args.mode == "grad-only":
         print(json.dumps({"differentiable": measure_gradient(torch, FastMultipoleMethod, ics, args)}), flush=True)
+        return 0
+    if args.mode == "mutual":
+        print(json.dumps(measure_mutual(torch, ics, args)), flush=True)
         return 0
 
     if args.calibrate_copy_mib > 0:
@@ -378,6 +384,73 @@ def main() -> int:
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def measure_mutual(torch, ics, args):
+    """The momentum-conserving block-step lane (jaccpot_amd.mutual) on a
+    Hernquist sphere (a=1, truncated at 100a, N=1,048,576, equal masses, seed
+    3), float64, order 4, theta 0.7, softening 1e-3, leaf ``--mutual-leaf``:
+    K full traversals (all levels, unit weights) and K fused boundary
+    traversals (rungs 0..3 from the acceleration magnitude, the weights of
+    sub-step boundary 1 at k_max 3), inputs and outputs resident on the GPU;
+    momentum residual and force error (4096 seeded targets against the
+    float64 direct sum) of the full traversal."""
+    from jaccpot_amd import direct_sum_gravitational_acceleration
+    from jaccpot_amd import mutual as mu
+
+    n, theta, order, eps, leaf, k_max = 1_048_576, 0.7, 4, 1e-3, args.mutual_leaf, 3
+    x, m = ics.galaxy(n, 3, halo_fraction=1.0)
+    t0 = time.perf_counter()
+    topo, _ = mu.build_mutual_topology(x, m, theta=theta, order=order, leaf_size=leaf)
+    st = mu.build_mutual_state(topo, softening=eps)
+    torch.cuda.synchronize()
+    prep_s = time.perf_counter() - t0
+    xd = torch.as_tensor(x, device="cuda")
+    md = torch.as_tensor(m, device="cuda")
+    acc = mu.mutual_accelerations(st, xd, md)
+    amag = torch.linalg.norm(acc, dim=1)
+    rung = torch.clamp(torch.floor(torch.log2(torch.clamp(amag / amag.median(), min=1.0))), 0, k_max).to(torch.int32)
+    w = mu.boundary_level_weights(1, k_max, 0.01)
+
+    def timed(fn):
+        for _ in range(max(1, args.warmup)):
+            fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(args.steps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / args.steps
+
+    ms_full = timed(lambda: mu.mutual_accelerations(st, xd, md))
+    ms_kick = timed(lambda: mu.mutual_weighted_accelerations(st, xd, md, rung=rung, level_weights=w))
+    acc = mu.mutual_accelerations(st, xd, md)
+    terms = md[:, None] * acc
+    momentum = float(torch.linalg.norm(terms.sum(0)) / torch.linalg.norm(terms.abs().sum(0)))
+    tgt = np.sort(np.random.default_rng(12345).choice(n, 4096, replace=False))
+    ref = direct_sum_gravitational_acceleration(xd, md, G=1.0, softening=eps, target_indices=tgt)
+    a_t = acc[torch.as_tensor(tgt, device="cuda")]
+    err = (torch.linalg.norm(a_t - ref, dim=1) / torch.linalg.norm(ref, dim=1)).cpu().numpy()
+    interactions = int(st.prepared.topology.p2p_interactions())
+    counts = np.bincount(rung.cpu().numpy(), minlength=k_max + 1).tolist()
+    log(f"mutual: full {ms_full:.3f} ms, fused boundary {ms_kick:.3f} ms, momentum {momentum:.2e}, "
+        f"median err {np.median(err):.2e}")
+    return {
+        "metric": "mutual-lane traversals at N=1M Hernquist, float64 (momentum-conserving, block-step weights)",
+        "value": n / (ms_full * 1e-3), "unit": "particles/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_full, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"Hernquist a=1 r<=100a N={n} seed 3, theta {theta}, order {order}, leaf {leaf}, "
+                               f"softening {eps}, float64", "parallelism": "single-gpu"},
+        "ms_fused_boundary": ms_kick, "rung_counts": counts, "prepare_s": prep_s,
+        "momentum_residual": momentum,
+        "force_error": {"median": float(np.median(err)), "p99": float(np.percentile(err, 99)),
+                        "targets": int(tgt.size)},
+        "near_interactions": interactions, "far_pairs_canonical": topo.num_far_pairs,
+        "near_pairs_canonical": topo.num_near_pairs,
+    }
 
 
 def measure_gradient(torch, FastMultipoleMethod, ics, args):

@@ -18,24 +18,12 @@
 #include "device_math.hpp"
 #include "launch.hpp"
 #include "rows.hpp"
+#include "xcd.hpp"
 
 namespace jfm {
 namespace {
 
 constexpr int kWave = 64;
-constexpr int kXcds = 8;
-
-// Workgroups are dealt round-robin to the eight XCDs; this maps launch order
-// to a logical block so that each XCD takes one contiguous run of blocks
-// (targets in tree order: neighbouring targets share most of their far-list
-// sources, which then stay in that XCD's L2).  Bijective: XCD x < r owns
-// q + 1 blocks, the others q, where nblk = 8 q + r.
-__device__ __forceinline__ int xcd_contiguous(int b, int nblk) {
-  const int q = nblk / kXcds, r = nblk % kXcds;
-  const int x = b % kXcds, slot = b / kXcds;
-  return x * q + min(x, r) + slot;
-}
-
 // coefficients per pass of the LDS reductions: a 64 x 33 buffer (8.4 KiB for
 // float32) keeps LDS from limiting the waves resident per CU
 constexpr int kChunk = 32;

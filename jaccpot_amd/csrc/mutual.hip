@@ -27,6 +27,7 @@
 
 #include "launch.hpp"
 #include "rows.hpp"
+#include "xcd.hpp"
 
 namespace jfm {
 namespace {
@@ -62,12 +63,21 @@ __global__ __launch_bounds__(256) void node_rungs_kernel(const int32_t* __restri
   node_rung[u] = max(node_rung[left[u]], node_rung[right[u]]);
 }
 
+// 1/sqrt(r2), 0 for r2 <= 0.  float32: v_rsq_f32 (about 1 ulp); float64:
+// v_rsq_f64 refined by one third-order step of the Newton iteration for
+// r^-1/2, y <- y + y e (1/2 + 3 e / 8) with e = 1 - r2 y^2 (the relative
+// error is cubed).  A deterministic function of r2 alone, so the two ends of
+// a pair get the same value.
 template <typename T>
 __device__ __forceinline__ T inv_sqrt(T r2) {
-  if constexpr (sizeof(T) == 4)
+  if constexpr (sizeof(T) == 4) {
     return r2 > T(0) ? __builtin_amdgcn_rsqf(r2) : T(0);
-  else
-    return r2 > T(0) ? T(1) / sqrt(r2) : T(0);
+  } else {
+    const T y = __builtin_amdgcn_rsq(r2);
+    const T e = fma(-(r2 * y), y, T(1));
+    const T z = fma(y * e, fma(e, T(0.375), T(0.5)), y);
+    return r2 > T(0) ? z : T(0);
+  }
 }
 
 template <typename T>
@@ -81,24 +91,34 @@ __device__ __forceinline__ void kahan_add(T& sum, T& comp, T x) {
 // One workgroup per target leaf.  Thread t holds target slot t mod WT
 // (WT = leaf width rounded up to a power of two) and the source partition
 // t / WT; sources (self leaf, then the near leaves, leaf_size slots each)
-// stream through a 256-entry LDS tile, partition q taking the tile entries
-// [q WT, (q + 1) WT).  Per-tile partials are folded with Kahan compensation
-// and the partitions are added in fixed order.
-template <typename T, bool WEIGHTED>
-__global__ __launch_bounds__(kMutualThreads) void mutual_p2p_kernel(
-    const T* __restrict__ pos4, int32_t n, int32_t leaf_size, int32_t wt_log2, const int32_t* __restrict__ near_off,
-    const int32_t* __restrict__ near_src, const int32_t* __restrict__ rung_sorted, const T* __restrict__ level_w,
-    int32_t nlev, T eps2, T G, const int32_t* __restrict__ out_index, T* __restrict__ force) {
-  __shared__ V4<T> sp[kMutualThreads];
-  __shared__ int32_t sj[kMutualThreads];
-  __shared__ int32_t sr[kMutualThreads];
+// stream through a kTile-entry LDS tile (kLoads loads per thread; the
+// next tile's positions and the near-list entries of the one after it are
+// fetched into registers during the current one), partition q taking the
+// tile entries q, q + NP, q + 2 NP, ... (NP = THREADS / WT partitions; the
+// partitions of a wave read neighbouring entries, so the LDS reads hit
+// distinct banks); the tile's memory then holds the partition sums.
+// Empty slots hold mass 0 at the origin and the self pair has d = 0, so
+// neither needs a branch: both add exactly zero.  Per-tile partials are
+// folded with Kahan compensation and the partitions are added in fixed order.
+template <typename T, bool WEIGHTED, int THREADS>
+__global__ __launch_bounds__(THREADS) void mutual_p2p_kernel(
+    const T* __restrict__ pos4, int32_t n, int32_t leaf_size, int32_t wt_log2, int32_t leaf_log2,
+    const int32_t* __restrict__ near_off, const int32_t* __restrict__ near_src,
+    const int32_t* __restrict__ rung_sorted, const T* __restrict__ level_w, int32_t nlev, T eps2, T G,
+    const int32_t* __restrict__ out_index, T* __restrict__ force) {
+  constexpr int kLoads = 2;
+  constexpr int kTile = THREADS * kLoads;
+  static_assert(kTile * 4 >= THREADS * 3, "the tile must hold the reduction");
+  __shared__ V4<T> sp[kTile];
+  __shared__ int32_t sr[WEIGHTED ? kTile : 1];
   __shared__ T lw[kMaxLevels];
-  __shared__ T red[kMutualThreads][3];
-  const int leaf = blockIdx.x;
+  T(*red)[3] = reinterpret_cast<T(*)[3]>(sp);  // the tile, reused for the partition sums
+  const int leaf = xcd_contiguous(blockIdx.x, gridDim.x);  // neighbouring leaves share an XCD's L2
   const int t = threadIdx.x;
   const int wt = 1 << wt_log2;
   const int slot = t & (wt - 1);
   const int part = t >> wt_log2;
+  const int np = THREADS >> wt_log2;  // source partitions
   const int i = leaf * leaf_size + slot;
   const bool live = slot < leaf_size && i < n;
   V4<T> xi{T(0), T(0), T(0), T(0)};
@@ -113,54 +133,74 @@ __global__ __launch_bounds__(kMutualThreads) void mutual_p2p_kernel(
   const int b = near_off[leaf];
   const int nsrc = (near_off[leaf + 1] - b + 1) * leaf_size;
   T fx = T(0), fy = T(0), fz = T(0), cx = T(0), cy = T(0), cz = T(0);
-  for (int base = 0; base < nsrc; base += kMutualThreads) {
-    __syncthreads();
-    {
-      const int s = base + t;
+  // two-stage fetch while a tile is in use: the positions of the next tile
+  // (from source indices resolved one tile earlier) and the near-list
+  // entries of the tile after it
+  V4<T> nx[kLoads];
+  int nr[kLoads];
+  int jl[kLoads];
+  auto resolve = [&](int base) {
+#pragma unroll
+    for (int q = 0; q < kLoads; ++q) {
+      const int s = base + q * THREADS + t;
       int j = -1;
       if (s < nsrc) {
-        const int q = s / leaf_size;
-        const int src_leaf = q == 0 ? leaf : near_src[b + q - 1];
-        const int jj = src_leaf * leaf_size + (s - q * leaf_size);
+        const int ql = leaf_log2 >= 0 ? s >> leaf_log2 : s / leaf_size;
+        const int src_leaf = ql == 0 ? leaf : near_src[b + ql - 1];
+        const int jj = src_leaf * leaf_size + (s - ql * leaf_size);
         if (jj < n) j = jj;
       }
-      sj[t] = j;
-      if (j >= 0) {
-        sp[t] = load4(pos4, j);
-        if constexpr (WEIGHTED) sr[t] = rung_sorted[j];
-      }
+      jl[q] = j;
+    }
+  };
+  auto fetch = [&]() {
+#pragma unroll
+    for (int q = 0; q < kLoads; ++q) {
+      nx[q] = jl[q] >= 0 ? load4(pos4, jl[q]) : V4<T>{T(0), T(0), T(0), T(0)};
+      if constexpr (WEIGHTED) nr[q] = jl[q] >= 0 ? rung_sorted[jl[q]] : 0;
+    }
+  };
+  resolve(0);
+  fetch();
+  resolve(kTile);
+  for (int base = 0; base < nsrc; base += kTile) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kLoads; ++q) {
+      sp[q * THREADS + t] = nx[q];
+      if constexpr (WEIGHTED) sr[q * THREADS + t] = nr[q];
     }
     __syncthreads();
-    if (live) {
-      T px = T(0), py = T(0), pz = T(0);
-      const int e0 = part * wt;
-      const int e1 = min(e0 + wt, nsrc - base);
-      for (int e = e0; e < e1; ++e) {
-        const int j = sj[e];
-        if (j < 0 || j == i) continue;
-        const V4<T> xj = sp[e];
-        const T dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-        const T r2 = fma(dz, dz, fma(dy, dy, fma(dx, dx, eps2)));
-        const T inv = inv_sqrt(r2);
-        const T inv3 = inv * inv * inv;
-        T c = xi.w * xj.w;
-        if constexpr (WEIGHTED) c = c * lw[min(max(max(ri, sr[e]), 0), nlev - 1)];
-        const T f = (c * inv3) * G;
-        px = fma(f, dx, px);
-        py = fma(f, dy, py);
-        pz = fma(f, dz, pz);
-      }
-      kahan_add(fx, cx, px);
-      kahan_add(fy, cy, py);
-      kahan_add(fz, cz, pz);
+    if (base + kTile < nsrc) {
+      fetch();
+      resolve(base + 2 * kTile);
     }
+    T px = T(0), py = T(0), pz = T(0);
+    const int lim = min(kTile, nsrc - base);
+#pragma unroll 4
+    for (int e = part; e < lim; e += np) {
+      const V4<T> xj = sp[e];
+      const T dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+      const T r2 = fma(dz, dz, fma(dy, dy, fma(dx, dx, eps2)));
+      const T inv = inv_sqrt(r2);
+      const T inv3 = inv * inv * inv;
+      T c = xi.w * xj.w;
+      if constexpr (WEIGHTED) c = c * lw[min(max(max(ri, sr[e]), 0), nlev - 1)];
+      const T f = (c * inv3) * G;
+      px = fma(f, dx, px);
+      py = fma(f, dy, py);
+      pz = fma(f, dz, pz);
+    }
+    kahan_add(fx, cx, px);
+    kahan_add(fy, cy, py);
+    kahan_add(fz, cz, pz);
   }
+  __syncthreads();  // every partition is done with the tile
   red[t][0] = fx;
   red[t][1] = fy;
   red[t][2] = fz;
   __syncthreads();
   if (live && part == 0) {
-    const int np = kMutualThreads >> wt_log2;
     T sx = T(0), sy = T(0), sz = T(0);
     for (int q = 0; q < np; ++q) {
       sx += red[q * wt + slot][0];
@@ -191,6 +231,27 @@ __global__ __launch_bounds__(256) void mutual_combine_kernel(int32_t n, const T*
 }
 
 int nblocks(int64_t n, int t) { return static_cast<int>((n + t - 1) / t); }
+
+template <typename T>
+struct MutualArgs {
+  const T* pos4;
+  int32_t n, leaf_size, wt_log2, leaf_log2;
+  const int32_t* near_off;
+  const int32_t* near_src;
+  const int32_t* rung_sorted;
+  const T* level_w;
+  int32_t nlev;
+  T eps2, G;
+  const int32_t* out_index;
+  T* force;
+};
+
+template <typename T, bool W, int TH>
+void launch_mutual(const MutualArgs<T>& a, int num_leaves, hipStream_t s) {
+  mutual_p2p_kernel<T, W, TH><<<num_leaves, TH, 0, s>>>(a.pos4, a.n, a.leaf_size, a.wt_log2, a.leaf_log2,
+                                                        a.near_off, a.near_src, a.rung_sorted, a.level_w, a.nlev,
+                                                        a.eps2, a.G, a.out_index, a.force);
+}
 
 }  // namespace
 }  // namespace jfm
@@ -230,16 +291,18 @@ int32_t jfm_mutual_p2p(void* stream, int32_t dtype, const void* pos4, int32_t n,
     return fail(JFM_ERR_ARG, "jfm_mutual_p2p: weights need rungs and 1..64 levels");
   int wt_log2 = 0;
   while ((1 << wt_log2) < leaf_size) ++wt_log2;
+  const int leaf_log2 = (1 << wt_log2) == leaf_size ? wt_log2 : -1;
   hipStream_t s = as_stream(stream);
   JFM_FOR_DTYPE(dtype, {
+    const T* lw = weighted ? (const T*)level_weights : nullptr;
+    const int32_t* rs = weighted ? rung_sorted : nullptr;
+    const int nl = weighted ? num_levels : 1;
+    const MutualArgs<T> a{(const T*)pos4, n, leaf_size, wt_log2, leaf_log2, near_off, near_src, rs, lw, nl,
+                          static_cast<T>(eps2), static_cast<T>(G), out_index, (T*)force};
     if (weighted)
-      mutual_p2p_kernel<T, true><<<num_leaves, kMutualThreads, 0, s>>>(
-          (const T*)pos4, n, leaf_size, wt_log2, near_off, near_src, rung_sorted, (const T*)level_weights, num_levels,
-          static_cast<T>(eps2), static_cast<T>(G), out_index, (T*)force);
+      launch_mutual<T, true, kMutualThreads>(a, num_leaves, s);
     else
-      mutual_p2p_kernel<T, false><<<num_leaves, kMutualThreads, 0, s>>>(
-          (const T*)pos4, n, leaf_size, wt_log2, near_off, near_src, nullptr, nullptr, 1, static_cast<T>(eps2),
-          static_cast<T>(G), out_index, (T*)force);
+      launch_mutual<T, false, kMutualThreads>(a, num_leaves, s);
   });
   return check_launch("jfm_mutual_p2p");
 }

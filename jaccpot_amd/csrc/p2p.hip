@@ -21,12 +21,12 @@
 #include "device_math.hpp"
 #include "launch.hpp"
 #include "rows.hpp"
+#include "xcd.hpp"
 
 namespace jfm {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kXcds = 8;
 // LDS per CU is 160 KiB; a workgroup holding just over 1/(N+1) of it caps the
 // CU at N workgroups (LDS is allocated in 512-byte granules).
 constexpr int kLdsPerCu = 160 * 1024;
@@ -38,11 +38,6 @@ constexpr int kP2PLdsBytes = (kLdsPerCu / (kP2PBlocksPerCu + 1) / 512 + 1) * 512
 static_assert(kP2PLdsBytes * (kP2PBlocksPerCu + 1) > kLdsPerCu && kP2PLdsBytes * kP2PBlocksPerCu <= kLdsPerCu,
               "P2P LDS reservation must cap the CU at kP2PBlocksPerCu workgroups");
 
-__device__ __forceinline__ int xcd_contiguous(int b, int nblk) {
-  const int q = nblk / kXcds, r = nblk % kXcds;
-  const int x = b % kXcds, slot = b / kXcds;
-  return x * q + min(x, r) + slot;
-}
 
 template <typename T>
 __device__ __forceinline__ T fast_rsqrt(T x) {

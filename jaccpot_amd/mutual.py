@@ -270,7 +270,6 @@ def build_mutual_state(topology: MutualTopology, *, softening: float, G: float =
         pallas_interpret=bool(pallas_interpret), num_particles_=n, num_near_pairs=topology.num_near_pairs,
         num_far_pairs=topology.num_far_pairs, flat=flat, level_off=np.asarray(prep.topology.m2m_off))
     st.buffers = {
-        "near": torch.empty((n, 3), dtype=prep.dtype, device=dev),
         "far_acc": torch.empty((n, 3), dtype=prep.dtype, device=dev),
         "rung_sorted": torch.empty(n, dtype=torch.int32, device=dev),
         "node_rung": torch.empty(prep.topology.num_nodes, dtype=torch.int32, device=dev),
@@ -369,7 +368,7 @@ def _evaluate(st: MutualFMMState, x: torch.Tensor, m: torch.Tensor, rung, level_
     far_acc = st.buffers["far_acc"]
     nat.call("jfm_l2p", s, code, p, prep.pos4.data_ptr(), n, ls, I, prep.locals.data_ptr(),
              prep.centers4.data_ptr(), st.G, prep.perm.data_ptr(), far_acc.data_ptr(), None, 0)
-    near = st.buffers["near"]
+    near = torch.empty((n, 3), dtype=prep.dtype, device=prep.device)
     nat.call("jfm_mutual_p2p", s, code, prep.pos4.data_ptr(), n, L, ls, prep.near_off.data_ptr(),
              prep.near_src.data_ptr(), st.buffers["rung_sorted"].data_ptr() if weighted else None,
              lw.data_ptr() if weighted else None, int(lw.numel()) if weighted else 0, st.softening ** 2, st.G,
@@ -378,7 +377,7 @@ def _evaluate(st: MutualFMMState, x: torch.Tensor, m: torch.Tensor, rung, level_
     far = torch.empty((n, 3), dtype=prep.dtype, device=prep.device)
     nat.call("jfm_mutual_combine", s, code, n, m.data_ptr(), far_acc.data_ptr(), near.data_ptr(), acc.data_ptr(),
              far.data_ptr())
-    return acc, near.clone(), far
+    return acc, near, far
 
 
 def mutual_weighted_accelerations(state: MutualFMMState, positions, masses, *, rung=None, level_weights=None,
