@@ -391,6 +391,22 @@ int32_t jfm_center_adjoint(void* stream, int32_t dtype, int32_t p, const int32_t
                            const int32_t* far_off, const int32_t* far_src, const void* multipoles,
                            const void* locals_own, const void* lbar_flip, const void* mbar_own_flip,
                            const void* centers4, void* cbar4);
+/* Mutual-lane reverse pass (level-weighted): jfm_p2p_adjoint with pair (i, j)
+ * scaled by level_weights[clamp(max(rung_sorted[i], rung_sorted[j]), 0,
+ * num_levels - 1)] (num_levels <= 64), and jfm_center_adjoint with far pair
+ * (u, v) scaled by the weight of max(node_rung[u], node_rung[v]); with them
+ * and jfm_m2l_flat_weighted (the transposed M2L on the symmetric list) the
+ * reverse sweep differentiates jfm_mutual_* forces (the reference's mutual
+ * lane under jax.grad, mutual/force.py:962-1068). */
+int32_t jfm_p2p_adjoint_weighted(void* stream, int32_t dtype, const void* pos4, const void* cot4, int32_t n,
+                                 int32_t num_leaves, int32_t leaf_size, const int32_t* near_off,
+                                 const int32_t* near_src, const int32_t* rung_sorted, const void* level_weights,
+                                 int32_t num_levels, double eps2, double G, void* grad4);
+int32_t jfm_center_adjoint_weighted(void* stream, int32_t dtype, int32_t p, const int32_t* nodes, int32_t num_nodes,
+                                    const int32_t* far_off, const int32_t* far_src, const int32_t* node_rung,
+                                    const void* level_weights, int32_t num_levels, const void* multipoles,
+                                    const void* locals_own, const void* lbar_flip, const void* mbar_own_flip,
+                                    const void* centers4, void* cbar4);
 /* jfm_center_adjoint for per-pair orders (learned gears): pair k = (u <- v) of
  * the far CSR ran at order gear_fwd[k], its mirror (v <- u) at gear_rev[k]
  * (orders <= p).  With it, Mbar_own of each gear g is jfm_m2l_gear at gear g

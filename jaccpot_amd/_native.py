@@ -28,7 +28,7 @@ MAX_ORDER = 8
 M2L_VALU = 0
 M2L_MFMA = 1
 M2L_ENGINES = {"valu": M2L_VALU, "mfma": M2L_MFMA}
-ABI_VERSION = 10
+ABI_VERSION = 11
 
 _i32p = POINTER(c_int32)
 _f64p = POINTER(c_double)
@@ -126,6 +126,12 @@ _SIGNATURES = {
                                         c_void_p, c_void_p]),
     "jfm_mutual_combine": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
                                      c_void_p]),
+    "jfm_p2p_adjoint_weighted": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_int32, c_int32,
+                                           c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_double, c_double,
+                                           c_void_p]),
+    "jfm_center_adjoint_weighted": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_int32, c_void_p, c_void_p,
+                                              c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
+                                              c_void_p, c_void_p]),
     "jfm_center_adjoint_gears": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_int32, c_void_p, c_void_p,
                                            c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                            c_void_p]),

@@ -142,17 +142,32 @@ struct AdjItems {
   int32_t num_part;
 };
 
-template <typename T, int WT, bool SAFE>
+// Block-step level weights of the mutual lane: pair (i, j) is scaled by
+// level_w[clamp(max(rung[i], rung[j]), 0, nlev - 1)] (rungs in tree order for
+// particles, per node for cells).
+template <typename T>
+struct PairLevels {
+  const int32_t* rung;
+  const T* level_w;
+  int32_t nlev;
+};
+
+constexpr int kMaxLevelWeights = 64;
+
+template <typename T, int WT, bool SAFE, bool WEIGHTED = false>
 __global__ __launch_bounds__(kThreads) void p2p_adjoint_kernel(const T* __restrict__ pos4, const T* __restrict__ cot4,
                                                                int32_t n, int32_t leaf_size,
                                                                const int32_t* __restrict__ near_off,
                                                                const int32_t* __restrict__ near_src, T eps2, T G,
-                                                               T* __restrict__ grad4, AdjItems<T> items) {
+                                                               T* __restrict__ grad4, AdjItems<T> items,
+                                                               PairLevels<T> lv = PairLevels<T>{}) {
   constexpr int NPART = kThreads / WT;
   constexpr int TILE = sizeof(T) == 4 ? 512 : 256;
   constexpr int LOADS = TILE / kThreads;
   static_assert(2 * kThreads <= 2 * TILE, "reduction buffer");
   __shared__ V4<T> tx[TILE], tc[TILE];
+  __shared__ int32_t tr[WEIGHTED ? TILE : 1];
+  __shared__ T lw[WEIGHTED ? kMaxLevelWeights : 1];
   const int leaf = items.leaf ? items.leaf[blockIdx.x] : blockIdx.x;
   const int i = threadIdx.x % WT, part = threadIdx.x / WT;
   const int first = leaf * leaf_size;
@@ -160,6 +175,11 @@ __global__ __launch_bounds__(kThreads) void p2p_adjoint_kernel(const T* __restri
   const int me = first + (i < cnt ? i : 0);
   const V4<T> xi = load4(pos4, me);
   const V4<T> ci = load4(cot4, me);
+  int ri = 0;
+  if constexpr (WEIGHTED) {
+    ri = lv.rung[me];
+    if (threadIdx.x < lv.nlev) lw[threadIdx.x] = lv.level_w[threadIdx.x];
+  }
   const int nb = near_off[leaf];
   const int start = items.leaf ? items.s0[blockIdx.x] : 0;
   const int total = items.leaf ? items.s1[blockIdx.x] : (1 + near_off[leaf + 1] - nb) * leaf_size;
@@ -170,6 +190,7 @@ __global__ __launch_bounds__(kThreads) void p2p_adjoint_kernel(const T* __restri
     for (int l = 0; l < LOADS; ++l) {
       const int slot = base + l * kThreads + threadIdx.x;
       V4<T> q{T(0), T(0), T(0), T(0)}, c{T(0), T(0), T(0), T(0)};
+      int r = 0;
       if (slot < total) {
         const int qi = slot / leaf_size;
         const int o = slot - qi * leaf_size;
@@ -178,10 +199,12 @@ __global__ __launch_bounds__(kThreads) void p2p_adjoint_kernel(const T* __restri
         if (g < n) {
           q = load4(pos4, g);
           c = load4(cot4, g);
+          if constexpr (WEIGHTED) r = lv.rung[g];
         }
       }
       tx[l * kThreads + threadIdx.x] = q;
       tc[l * kThreads + threadIdx.x] = c;
+      if constexpr (WEIGHTED) tr[l * kThreads + threadIdx.x] = r;
     }
     __syncthreads();
     const int trips = (min(TILE, total - base) + NPART - 1) / NPART;
@@ -197,8 +220,14 @@ __global__ __launch_bounds__(kThreads) void p2p_adjoint_kernel(const T* __restri
       T sv = fast_rsqrt(r2);
       if (SAFE) sv = r2 > T(0) ? sv : T(0);
       const T sv2 = sv * sv;
-      const T s3 = sv2 * sv;
-      const T s5 = s3 * sv2;
+      T s3 = sv2 * sv;
+      T s5 = s3 * sv2;
+      if constexpr (WEIGHTED) {  // every term is linear in (s, s3, s5)
+        const T w = lw[min(max(max(ri, tr[part + k * NPART]), 0), lv.nlev - 1)];
+        sv *= w;
+        s3 *= w;
+        s5 *= w;
+      }
       // u = m_i abar_j - m_j abar_i
       const T ux = fma(xi.w, c.x, -s.w * ci.x), uy = fma(xi.w, c.y, -s.w * ci.y), uz = fma(xi.w, c.z, -s.w * ci.z);
       const T du = fma(dx, ux, fma(dy, uy, dz * uz));
@@ -387,12 +416,14 @@ __device__ __forceinline__ void m2l_bilinear_grad_at(int g, const T* A, const T*
 // (v <- u) at gear_rev[k]; each bilinear term runs at its pair's order (the
 // order-g operator keeps the terms n + k <= g, so it is not the order-P one
 // with rows cut at degree g).
-template <typename T, int P, bool GEARS>
+// WEIGHTED: both terms of pair (u, v) are scaled by its level weight (the
+// mutual lane; L_own and Mbar_own then carry the weights already).
+template <typename T, int P, bool GEARS, bool WEIGHTED = false>
 __global__ __launch_bounds__(kWave) void center_adjoint_kernel(
     const int32_t* __restrict__ nodes, const int32_t* __restrict__ far_off, const int32_t* __restrict__ far_src,
     const int8_t* __restrict__ gear_fwd, const int8_t* __restrict__ gear_rev, const T* __restrict__ mult,
     const T* __restrict__ lown, const T* __restrict__ lbar_flip, const T* __restrict__ mbar_own_flip, int32_t cs,
-    const T* __restrict__ centers4, T* __restrict__ cbar4) {
+    const T* __restrict__ centers4, T* __restrict__ cbar4, PairLevels<T> lv = PairLevels<T>{}) {
   constexpr int C = ncoef(P);
   __shared__ T red[kWave][4];
   const int u = nodes[blockIdx.x];
@@ -428,9 +459,16 @@ __global__ __launch_bounds__(kWave) void center_adjoint_kernel(
       m2l_bilinear_grad<T, P>(Lu, Mv, dx, dy, dz, ax, ay, az);
       m2l_bilinear_grad<T, P>(Lv, Mu, -dx, -dy, -dz, bx, by, bz);
     }
-    gx += ax - bx;
-    gy += ay - by;
-    gz += az - bz;
+    if constexpr (WEIGHTED) {
+      const T w = lv.level_w[min(max(max(lv.rung[u], lv.rung[v]), 0), lv.nlev - 1)];
+      gx = fma(w, ax - bx, gx);
+      gy = fma(w, ay - by, gy);
+      gz = fma(w, az - bz, gz);
+    } else {
+      gx += ax - bx;
+      gy += ay - by;
+      gz += az - bz;
+    }
   }
   red[lane][0] = gx;
   red[lane][1] = gy;
@@ -547,17 +585,23 @@ __global__ __launch_bounds__(kThreads) void leaf_adjoint_b_kernel(const T* __res
 template <typename T>
 int32_t launch_p2p_adj(hipStream_t s, const T* pos4, const T* cot4, int32_t n, int32_t leaf_size,
                        int32_t num_blocks, const int32_t* near_off, const int32_t* near_src, T eps2, T G, T* grad4,
-                       AdjItems<T> items = AdjItems<T>{}) {
+                       AdjItems<T> items = AdjItems<T>{}, PairLevels<T> lv = PairLevels<T>{}) {
   const bool safe = !(eps2 > T(0));
-#define JFM_ADJ(WT)                                                                                            \
-  {                                                                                                            \
-    if (safe)                                                                                                  \
-      p2p_adjoint_kernel<T, WT, true><<<num_blocks, kThreads, 0, s>>>(pos4, cot4, n, leaf_size, near_off,      \
-                                                                      near_src, eps2, G, grad4, items);        \
-    else                                                                                                       \
-      p2p_adjoint_kernel<T, WT, false><<<num_blocks, kThreads, 0, s>>>(pos4, cot4, n, leaf_size, near_off,     \
-                                                                       near_src, eps2, G, grad4, items);       \
-    return check_launch("jfm_p2p_adjoint");                                                                    \
+  const bool weighted = lv.level_w != nullptr;
+#define JFM_ADJ_W(WT, SAFE, W)                                                                                   \
+  p2p_adjoint_kernel<T, WT, SAFE, W><<<num_blocks, kThreads, 0, s>>>(pos4, cot4, n, leaf_size, near_off, near_src, \
+                                                                     eps2, G, grad4, items, lv)
+#define JFM_ADJ(WT)                 \
+  {                                 \
+    if (weighted && safe)           \
+      JFM_ADJ_W(WT, true, true);    \
+    else if (weighted)              \
+      JFM_ADJ_W(WT, false, true);   \
+    else if (safe)                  \
+      JFM_ADJ_W(WT, true, false);   \
+    else                            \
+      JFM_ADJ_W(WT, false, false);  \
+    return check_launch("jfm_p2p_adjoint"); \
   }
   if (leaf_size <= 16) JFM_ADJ(16)
   if (leaf_size <= 32) JFM_ADJ(32)
@@ -565,6 +609,7 @@ int32_t launch_p2p_adj(hipStream_t s, const T* pos4, const T* cot4, int32_t n, i
   if (leaf_size <= 128) JFM_ADJ(128)
   JFM_ADJ(256)
 #undef JFM_ADJ
+#undef JFM_ADJ_W
 }
 
 inline int nblk(int64_t n, int b) { return static_cast<int>((n + b - 1) / b); }
@@ -672,6 +717,49 @@ int32_t jfm_center_adjoint(void* stream, int32_t dtype, int32_t p, const int32_t
         (const T*)lbar_flip, (const T*)mbar_own_flip, coef_stride(P, sizeof(T)), (const T*)centers4, (T*)cbar4);
   }));
   return check_launch("jfm_center_adjoint");
+}
+
+int32_t jfm_center_adjoint_weighted(void* stream, int32_t dtype, int32_t p, const int32_t* nodes, int32_t num_nodes,
+                                    const int32_t* far_off, const int32_t* far_src, const int32_t* node_rung,
+                                    const void* level_weights, int32_t num_levels, const void* multipoles,
+                                    const void* locals_own, const void* lbar_flip, const void* mbar_own_flip,
+                                    const void* centers4, void* cbar4) {
+  if (num_nodes < 0 || num_levels < 1) return fail(JFM_ERR_ARG, "jfm_center_adjoint_weighted: bad sizes");
+  if (num_nodes == 0) return JFM_OK;
+  if (!nodes || !far_off || !far_src || !node_rung || !level_weights || !multipoles || !locals_own || !lbar_flip ||
+      !mbar_own_flip || !centers4 || !cbar4)
+    return fail(JFM_ERR_ARG, "jfm_center_adjoint_weighted: NULL buffer");
+  hipStream_t s = as_stream(stream);
+  JFM_FOR_DTYPE(dtype, JFM_FOR_ADJ_ORDER(p, {
+    const PairLevels<T> lv{node_rung, (const T*)level_weights, num_levels};
+    center_adjoint_kernel<T, P, false, true><<<num_nodes, kWave, 0, s>>>(
+        nodes, far_off, far_src, nullptr, nullptr, (const T*)multipoles, (const T*)locals_own, (const T*)lbar_flip,
+        (const T*)mbar_own_flip, coef_stride(P, sizeof(T)), (const T*)centers4, (T*)cbar4, lv);
+  }));
+  return check_launch("jfm_center_adjoint_weighted");
+}
+
+int32_t jfm_p2p_adjoint_weighted(void* stream, int32_t dtype, const void* pos4, const void* cot4, int32_t n,
+                                 int32_t num_leaves, int32_t leaf_size, const int32_t* near_off,
+                                 const int32_t* near_src, const int32_t* rung_sorted, const void* level_weights,
+                                 int32_t num_levels, double eps2, double G, void* grad4) {
+  if (n < 1 || leaf_size < 1 || leaf_size > 256 || num_leaves != (n + leaf_size - 1) / leaf_size)
+    return fail(JFM_ERR_ARG, "jfm_p2p_adjoint_weighted: inconsistent sizes");
+  if (num_levels < 1 || num_levels > kMaxLevelWeights)
+    return fail(JFM_ERR_ARG, "jfm_p2p_adjoint_weighted: 1..64 level weights");
+  if (!pos4 || !cot4 || !near_off || !near_src || !rung_sorted || !level_weights || !grad4)
+    return fail(JFM_ERR_ARG, "jfm_p2p_adjoint_weighted: NULL buffer");
+  hipStream_t s = as_stream(stream);
+  if (dtype == JFM_F32)
+    return launch_p2p_adj<float>(s, (const float*)pos4, (const float*)cot4, n, leaf_size, num_leaves, near_off,
+                                 near_src, static_cast<float>(eps2), static_cast<float>(G), (float*)grad4,
+                                 AdjItems<float>{}, PairLevels<float>{rung_sorted, (const float*)level_weights,
+                                                                      num_levels});
+  if (dtype == JFM_F64)
+    return launch_p2p_adj<double>(s, (const double*)pos4, (const double*)cot4, n, leaf_size, num_leaves, near_off,
+                                  near_src, eps2, G, (double*)grad4, AdjItems<double>{},
+                                  PairLevels<double>{rung_sorted, (const double*)level_weights, num_levels});
+  return fail(JFM_ERR_ARG, "jfm_p2p_adjoint_weighted: bad dtype");
 }
 
 int32_t jfm_center_adjoint_gears(void* stream, int32_t dtype, int32_t p, const int32_t* nodes, int32_t num_nodes,

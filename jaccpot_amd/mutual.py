@@ -20,7 +20,9 @@ topology.py, force.py) and of its block-step adapter
 * forces are accumulated and divided by the mass once (``jfm_mutual_combine``).
 
 Everything runs at a frozen topology on live positions and masses, as the
-reference's mutual state does.  Gradients are not provided on this lane.
+reference's mutual state does; torch inputs that require grad are
+differentiated by the fixed-order adjoint kernels with the level weights on
+every pair (positions and masses; topology, rungs and weights held fixed).
 """
 
 from __future__ import annotations
@@ -303,72 +305,100 @@ def _output(t: torch.Tensor, kind):
     return t.to(device=kind[2], dtype=dt)
 
 
-def _evaluate(st: MutualFMMState, x: torch.Tensor, m: torch.Tensor, rung, level_weights):
+def _levels(st: MutualFMMState, rung, level_weights):
+    """Level weights on the device and the particle / cell rungs (None when unweighted)."""
+    if level_weights is None:
+        return None
+    if rung is None:
+        raise ValueError("level_weights need a per-particle rung")
+    prep = st.prepared
+    topo = prep.topology
+    s = nat.current_stream()
+    lw = torch.as_tensor(np.asarray(level_weights, np.float64) if not isinstance(level_weights, torch.Tensor)
+                         else level_weights.detach()).to(prep.device, prep.dtype).reshape(-1).contiguous()
+    if not 1 <= lw.numel() <= MAX_LEVELS:
+        raise ValueError(f"level_weights must have 1..{MAX_LEVELS} entries")
+    r = torch.as_tensor(np.asarray(rung) if not isinstance(rung, torch.Tensor) else rung)
+    r = r.to(prep.device, torch.int32).reshape(-1).contiguous()
+    if r.numel() != topo.num_particles:
+        raise ValueError(f"rung must have {topo.num_particles} entries")
+    nat.call("jfm_mutual_rungs", s, topo.num_particles, topo.num_leaves, topo.leaf_size, prep.perm.data_ptr(),
+             r.data_ptr(), st.buffers["rung_sorted"].data_ptr(), st.buffers["node_rung"].data_ptr())
+    off = st.level_off
+    for lev in range(len(off) - 1):  # deepest level first
+        a, b = int(off[lev]), int(off[lev + 1])
+        if b > a:
+            nat.call("jfm_mutual_rungs_level", s, prep.m2m_nodes.data_ptr() + 4 * a, b - a, prep.left.data_ptr(),
+                     prep.right.data_ptr(), st.buffers["node_rung"].data_ptr())
+    return lw
+
+
+def _tree_level(prep, nodes: torch.Tensor, off, op: str, rows: torch.Tensor, code: int, s) -> None:
+    """One pass of M2M (deepest level first) or L2L (root first) over ``rows``."""
+    for lev in range(len(off) - 1):
+        a, b = int(off[lev]), int(off[lev + 1])
+        if b <= a:
+            continue
+        if op == "m2m":
+            nat.call("jfm_m2m_level", s, code, prep.order, nodes.data_ptr() + 4 * a, b - a, prep.left.data_ptr(),
+                     prep.right.data_ptr(), prep.node_range.data_ptr(), rows.data_ptr(), prep.centers4.data_ptr())
+        else:
+            nat.call("jfm_l2l_level", s, code, prep.order, nodes.data_ptr() + 4 * a, b - a, prep.parent.data_ptr(),
+                     prep.centers4.data_ptr(), rows.data_ptr())
+
+
+def _far_m2l(st: MutualFMMState, lw, src: torch.Tensor, dst: torch.Tensor, code: int, s) -> None:
+    """dst += M2L of src over both directions of every far pair (level-weighted when lw is given).
+    On the symmetric list with symmetric weights this is also the transposed operator."""
+    fl = st.flat
+    if fl is None or not fl.num_pairs:
+        return
+    p = st.prepared.order
+    c4 = st.prepared.centers4
+    if lw is not None:
+        nat.call("jfm_m2l_flat_weighted", s, code, p, fl.num_pairs, fl.pair_tgt.data_ptr(), fl.pair_src.data_ptr(),
+                 st.buffers["node_rung"].data_ptr(), lw.data_ptr(), int(lw.numel()), src.data_ptr(), c4.data_ptr(),
+                 dst.data_ptr(), fl.scratch.data_ptr(), fl.num_span, fl.span_tgt.data_ptr(), fl.span_w0.data_ptr(),
+                 fl.span_w1.data_ptr())
+    else:
+        nat.call("jfm_m2l_flat", s, code, p, p, fl.num_pairs, fl.pair_tgt.data_ptr(), fl.pair_src.data_ptr(),
+                 src.data_ptr(), c4.data_ptr(), dst.data_ptr(), fl.scratch.data_ptr(), fl.num_span,
+                 fl.span_tgt.data_ptr(), fl.span_w0.data_ptr(), fl.span_w1.data_ptr())
+
+
+def _expansions(st: MutualFMMState, x: torch.Tensor, m: torch.Tensor, lw) -> torch.Tensor:
+    """Gather, P2M, M2M, (weighted) M2L, L2L at the live inputs; returns the
+    locals after M2L alone (the full locals stay in the prepared state)."""
+    prep = st.prepared
+    topo = prep.topology
+    s = nat.current_stream()
+    code = nat.dtype_code(prep.dtype)
+    n = topo.num_particles
+    nat.call("jfm_gather_sorted", s, code, code, x.data_ptr(), m.data_ptr(), prep.perm.data_ptr(), n,
+             prep.pos4.data_ptr())
+    nat.call("jfm_p2m", s, code, prep.order, prep.pos4.data_ptr(), n, topo.num_leaves, topo.leaf_size,
+             topo.num_internal, prep.multipoles.data_ptr(), prep.centers4.data_ptr())
+    _tree_level(prep, prep.m2m_nodes, topo.m2m_off, "m2m", prep.multipoles, code, s)
+    prep.locals.zero_()
+    _far_m2l(st, lw, prep.multipoles, prep.locals, code, s)
+    lown = prep.locals.clone()
+    _tree_level(prep, prep.l2l_nodes, topo.l2l_off, "l2l", prep.locals, code, s)
+    return lown
+
+
+def _evaluate(st: MutualFMMState, x: torch.Tensor, m: torch.Tensor, lw):
     """(acc, near_force, far_force) device tensors in input order."""
     prep = st.prepared
     topo = prep.topology
     s = nat.current_stream()
     code = nat.dtype_code(prep.dtype)
     n, L, ls, I = topo.num_particles, topo.num_leaves, topo.leaf_size, topo.num_internal
-    p = prep.order
-    weighted = level_weights is not None
-    if weighted and rung is None:
-        raise ValueError("level_weights need a per-particle rung")
-    lw = None
-    if weighted:
-        lw = torch.as_tensor(np.asarray(level_weights, np.float64) if not isinstance(level_weights, torch.Tensor)
-                             else level_weights).to(prep.device, prep.dtype).reshape(-1).contiguous()
-        if not 1 <= lw.numel() <= MAX_LEVELS:
-            raise ValueError(f"level_weights must have 1..{MAX_LEVELS} entries")
-        r = torch.as_tensor(np.asarray(rung) if not isinstance(rung, torch.Tensor) else rung)
-        r = r.to(prep.device, torch.int32).reshape(-1).contiguous()
-        if r.numel() != n:
-            raise ValueError(f"rung must have {n} entries")
-        nat.call("jfm_mutual_rungs", s, n, L, ls, prep.perm.data_ptr(), r.data_ptr(),
-                 st.buffers["rung_sorted"].data_ptr(), st.buffers["node_rung"].data_ptr())
-        off = st.level_off
-        for lev in range(len(off) - 1):  # deepest level first
-            a, b = int(off[lev]), int(off[lev + 1])
-            if b > a:
-                nat.call("jfm_mutual_rungs_level", s, prep.m2m_nodes.data_ptr() + 4 * a, b - a,
-                         prep.left.data_ptr(), prep.right.data_ptr(), st.buffers["node_rung"].data_ptr())
-    # upward at the live inputs
-    nat.call("jfm_gather_sorted", s, code, code, x.data_ptr(), m.data_ptr(), prep.perm.data_ptr(), n,
-             prep.pos4.data_ptr())
-    nat.call("jfm_p2m", s, code, p, prep.pos4.data_ptr(), n, L, ls, I, prep.multipoles.data_ptr(),
-             prep.centers4.data_ptr())
-    off = topo.m2m_off
-    for lev in range(len(off) - 1):
-        a, b = int(off[lev]), int(off[lev + 1])
-        if b > a:
-            nat.call("jfm_m2m_level", s, code, p, prep.m2m_nodes.data_ptr() + 4 * a, b - a, prep.left.data_ptr(),
-                     prep.right.data_ptr(), prep.node_range.data_ptr(), prep.multipoles.data_ptr(),
-                     prep.centers4.data_ptr())
-    # far field: both directions of every pair from the symmetric list
-    prep.locals.zero_()
-    fl = st.flat
-    if fl is not None and fl.num_pairs:
-        if weighted:
-            nat.call("jfm_m2l_flat_weighted", s, code, p, fl.num_pairs, fl.pair_tgt.data_ptr(),
-                     fl.pair_src.data_ptr(), st.buffers["node_rung"].data_ptr(), lw.data_ptr(), int(lw.numel()),
-                     prep.multipoles.data_ptr(), prep.centers4.data_ptr(), prep.locals.data_ptr(),
-                     fl.scratch.data_ptr(), fl.num_span, fl.span_tgt.data_ptr(), fl.span_w0.data_ptr(),
-                     fl.span_w1.data_ptr())
-        else:
-            nat.call("jfm_m2l_flat", s, code, p, p, fl.num_pairs, fl.pair_tgt.data_ptr(), fl.pair_src.data_ptr(),
-                     prep.multipoles.data_ptr(), prep.centers4.data_ptr(), prep.locals.data_ptr(),
-                     fl.scratch.data_ptr(), fl.num_span, fl.span_tgt.data_ptr(), fl.span_w0.data_ptr(),
-                     fl.span_w1.data_ptr())
-    off = topo.l2l_off
-    for lev in range(len(off) - 1):
-        a, b = int(off[lev]), int(off[lev + 1])
-        if b > a:
-            nat.call("jfm_l2l_level", s, code, p, prep.l2l_nodes.data_ptr() + 4 * a, b - a, prep.parent.data_ptr(),
-                     prep.centers4.data_ptr(), prep.locals.data_ptr())
+    _expansions(st, x, m, lw)
     far_acc = st.buffers["far_acc"]
-    nat.call("jfm_l2p", s, code, p, prep.pos4.data_ptr(), n, ls, I, prep.locals.data_ptr(),
+    nat.call("jfm_l2p", s, code, prep.order, prep.pos4.data_ptr(), n, ls, I, prep.locals.data_ptr(),
              prep.centers4.data_ptr(), st.G, prep.perm.data_ptr(), far_acc.data_ptr(), None, 0)
     near = torch.empty((n, 3), dtype=prep.dtype, device=prep.device)
+    weighted = lw is not None
     nat.call("jfm_mutual_p2p", s, code, prep.pos4.data_ptr(), n, L, ls, prep.near_off.data_ptr(),
              prep.near_src.data_ptr(), st.buffers["rung_sorted"].data_ptr() if weighted else None,
              lw.data_ptr() if weighted else None, int(lw.numel()) if weighted else 0, st.softening ** 2, st.G,
@@ -380,13 +410,112 @@ def _evaluate(st: MutualFMMState, x: torch.Tensor, m: torch.Tensor, rung, level_
     return acc, near, far
 
 
+def _vjp(st: MutualFMMState, x: torch.Tensor, m: torch.Tensor, lw, acc_bar: torch.Tensor):
+    """(positions_bar, masses_bar) in input order for the cotangent of the mutual
+    accelerations.  For m_i > 0 the acceleration is the level-weighted FMM
+    acceleration, so the reverse sweep is the fixed-order adjoint
+    (jaccpot_amd.autodiff) with every near and far pair carrying its weight;
+    rows of zero mass are constant zero and pass no cotangent."""
+    prep = st.prepared
+    topo = prep.topology
+    s = nat.current_stream()
+    code = nat.dtype_code(prep.dtype)
+    n, L, ls, I = topo.num_particles, topo.num_leaves, topo.leaf_size, topo.num_internal
+    dt, dev = prep.dtype, prep.device
+    lown = _expansions(st, x, m, lw)
+    ab = torch.where((m > 0)[:, None], acc_bar.to(dev, dt), torch.zeros((), dtype=dt, device=dev)).contiguous()
+    pb = torch.zeros(n, dtype=dt, device=dev)
+    cot4 = torch.empty((n, 4), dtype=dt, device=dev)
+    nat.call("jfm_gather_sorted", s, code, code, ab.data_ptr(), pb.data_ptr(), prep.perm.data_ptr(), n,
+             cot4.data_ptr())
+    grad4 = torch.empty((n, 4), dtype=dt, device=dev)
+    eps2 = st.softening ** 2
+    if lw is not None:
+        nat.call("jfm_p2p_adjoint_weighted", s, code, prep.pos4.data_ptr(), cot4.data_ptr(), n, L, ls,
+                 prep.near_off.data_ptr(), prep.near_src.data_ptr(), st.buffers["rung_sorted"].data_ptr(),
+                 lw.data_ptr(), int(lw.numel()), eps2, st.G, grad4.data_ptr())
+    else:
+        nat.call("jfm_p2p_adjoint", s, code, prep.pos4.data_ptr(), cot4.data_ptr(), n, L, ls,
+                 prep.near_off.data_ptr(), prep.near_src.data_ptr(), eps2, st.G, grad4.data_ptr())
+    lbar = torch.zeros_like(prep.locals)
+    nat.call("jfm_leaf_adjoint_a", s, code, prep.order, prep.pos4.data_ptr(), cot4.data_ptr(), n, ls, I,
+             prep.locals.data_ptr(), prep.centers4.data_ptr(), st.G, grad4.data_ptr(), lbar.data_ptr())
+    _tree_level(prep, prep.m2m_nodes, topo.m2m_off, "m2m", lbar, code, s)  # l2l^T
+    mbar = torch.zeros_like(prep.multipoles)
+    cbar4 = torch.zeros((topo.num_nodes, 4), dtype=dt, device=dev)
+    if topo.m2l_targets.size:
+        _far_m2l(st, lw, lbar, mbar, code, s)  # m2l^T on the symmetric list
+        nt = int(topo.m2l_targets.size)
+        if lw is not None:
+            nat.call("jfm_center_adjoint_weighted", s, code, prep.order, prep.m2l_targets.data_ptr(), nt,
+                     prep.far_off.data_ptr(), prep.far_src.data_ptr(), st.buffers["node_rung"].data_ptr(),
+                     lw.data_ptr(), int(lw.numel()), prep.multipoles.data_ptr(), lown.data_ptr(), lbar.data_ptr(),
+                     mbar.data_ptr(), prep.centers4.data_ptr(), cbar4.data_ptr())
+        else:
+            nat.call("jfm_center_adjoint", s, code, prep.order, prep.m2l_targets.data_ptr(), nt,
+                     prep.far_off.data_ptr(), prep.far_src.data_ptr(), prep.multipoles.data_ptr(), lown.data_ptr(),
+                     lbar.data_ptr(), mbar.data_ptr(), prep.centers4.data_ptr(), cbar4.data_ptr())
+    _tree_level(prep, prep.l2l_nodes, topo.l2l_off, "l2l", mbar, code, s)  # m2m^T
+    chain = torch.empty((L, 8), dtype=dt, device=dev)
+    nat.call("jfm_com_chain", s, code, L, I, prep.parent.data_ptr(), prep.node_range.data_ptr(),
+             prep.centers4.data_ptr(), cbar4.data_ptr(), chain.data_ptr())
+    gpos = torch.empty((n, 3), dtype=dt, device=dev)
+    gmass = torch.empty((n,), dtype=dt, device=dev)
+    nat.call("jfm_leaf_adjoint_b", s, code, prep.order, prep.pos4.data_ptr(), n, ls, I, mbar.data_ptr(),
+             prep.centers4.data_ptr(), chain.data_ptr(), grad4.data_ptr(), prep.perm.data_ptr(), gpos.data_ptr(),
+             gmass.data_ptr())
+    return gpos, gmass
+
+
+class _MutualForce(torch.autograd.Function):
+    """torch.autograd over the mutual accelerations (the reference's jax.grad
+    over mutual_weighted_accelerations, at fixed topology, rungs and weights)."""
+
+    @staticmethod
+    def forward(ctx, positions, masses, st, lw):
+        x = positions.detach().contiguous()
+        m = masses.detach().contiguous()
+        acc, _, _ = _evaluate(st, x, m, lw)
+        ctx.st, ctx.lw = st, lw
+        # the rung buffers are shared by every evaluation of the state
+        ctx.rungs = (st.buffers["rung_sorted"].clone(), st.buffers["node_rung"].clone()) if lw is not None else None
+        ctx.save_for_backward(x, m)
+        return acc
+
+    @staticmethod
+    def backward(ctx, gacc):
+        x, m = ctx.saved_tensors
+        if ctx.rungs is not None:
+            ctx.st.buffers["rung_sorted"].copy_(ctx.rungs[0])
+            ctx.st.buffers["node_rung"].copy_(ctx.rungs[1])
+        gx, gm = _vjp(ctx.st, x, m, ctx.lw, gacc.contiguous())
+        return gx, gm, None, None
+
+
 def mutual_weighted_accelerations(state: MutualFMMState, positions, masses, *, rung=None, level_weights=None,
                                   return_parts: bool = False):
     """``sum_k level_weights[k] a_k`` in one traversal (force.py:985-1068); with
     ``level_weights=None`` the full acceleration.  Inputs and outputs in the
-    caller's order; zero-mass particles get zero acceleration."""
+    caller's order; zero-mass particles get zero acceleration.  Torch inputs
+    that require grad get gradients through the adjoint kernels (the
+    accelerations; the near / far parts are returned detached)."""
+    lw = _levels(state, rung, level_weights)
+    grad = any(isinstance(t, torch.Tensor) and t.requires_grad for t in (positions, masses))
+    if grad:
+        prep = state.prepared
+        kind = ("torch", positions.dtype, positions.device)
+        x = positions.to(prep.device, prep.dtype)
+        m = torch.as_tensor(masses).to(prep.device, prep.dtype)
+        n = state.num_particles
+        if tuple(x.shape) != (n, 3) or tuple(m.shape) != (n,):
+            raise ValueError(f"positions/masses must have shapes ({n}, 3)/({n},) for this state")
+        acc = _MutualForce.apply(x, m, state, lw)
+        if return_parts:
+            _, near, far = _evaluate(state, x.detach().contiguous(), m.detach().contiguous(), lw)
+            return MutualForceResult(_output(acc, kind), _output(near, kind), _output(far, kind))
+        return _output(acc, kind)
     x, m, kind = _inputs(state, positions, masses)
-    acc, near, far = _evaluate(state, x, m, rung, level_weights)
+    acc, near, far = _evaluate(state, x, m, lw)
     if return_parts:
         return MutualForceResult(_output(acc, kind), _output(near, kind), _output(far, kind))
     return _output(acc, kind)

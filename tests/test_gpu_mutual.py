@@ -241,3 +241,89 @@ def test_fp32_momentum_at_single_precision_round_off(require_gpu):
     res = om.momentum_residual(acc, m.astype(np.float64))
     print(f"[gpu] mutual fp32 momentum residual {res:.2e}", flush=True)
     assert acc.dtype == np.float32 and res < 1e-6
+
+
+@pytest.mark.parametrize("n, leaf", [(1, 16), (5, 16), (40, 64)], ids=["one", "single_leaf", "below_leaf"])
+def test_small_systems(n, leaf, require_gpu):
+    """A single particle feels nothing; one leaf is the direct sum."""
+    from jaccpot_amd import mutual as mu
+
+    x, m = _system(n, 30 + n)
+    topo, _ = mu.build_mutual_topology(x, m, theta=0.6, order=3, leaf_size=leaf)
+    assert topo.num_far_pairs == 0 and topo.num_near_pairs == 0
+    st = mu.build_mutual_state(topo, softening=SOFT)
+    acc = mu.mutual_accelerations(st, x, m)
+    if n == 1:
+        assert np.all(acc == 0.0)
+    else:
+        assert _rel(acc, om.level_direct_sum(x, m, eps=SOFT)) < 1e-13
+        assert om.momentum_residual(acc, m) < 1e-15
+
+
+def _mutual_loss_grad(mu, st, x, m, w, rung, lw):
+    xt = torch.as_tensor(x, device="cuda").requires_grad_(True)
+    mt = torch.as_tensor(m, device="cuda").requires_grad_(True)
+    acc = mu.mutual_weighted_accelerations(st, xt, mt, rung=rung, level_weights=lw)
+    (acc * torch.as_tensor(w, device="cuda")).sum().backward()
+    return xt.grad.double().cpu().numpy(), mt.grad.double().cpu().numpy()
+
+
+def _mutual_loss(mu, st, x, m, w, rung, lw):
+    a = mu.mutual_weighted_accelerations(st, x, m, rung=rung, level_weights=lw)
+    return float((np.asarray(a, np.float64) * w).sum())
+
+
+@pytest.mark.parametrize("weighted", [False, True], ids=["unit", "levels"])
+def test_gradient_matches_central_differences(weighted, require_gpu):
+    """Reverse mode of the mutual accelerations (fixed topology, rungs and
+    weights) against float64 central differences."""
+    from jaccpot_amd import mutual as mu
+
+    n = 1500
+    print(f"[gpu] mutual gradient fd ({'levels' if weighted else 'unit weights'})", flush=True)
+    x, m, topo, st = _build(n, theta=0.8, order=5, leaf=16, seed=12)
+    assert topo.num_far_pairs > 0
+    rng = np.random.default_rng(13)
+    w = rng.normal(size=(n, 3))
+    rung = _rungs(n, 14) if weighted else None
+    lw = np.array([1.0, 0.5, 0.25, 0.125]) if weighted else None
+    gx, gm = _mutual_loss_grad(mu, st, x, m, w, rung, lw)
+    fds, ans = [], []
+    for k in range(6):
+        vx = rng.normal(size=x.shape)
+        vm = rng.normal(size=m.shape) * np.abs(m).mean()
+        h = 1e-6
+        fd = (_mutual_loss(mu, st, x + h * vx, m + h * vm, w, rung, lw)
+              - _mutual_loss(mu, st, x - h * vx, m - h * vm, w, rung, lw)) / (2 * h)
+        an = float((gx * vx).sum() + (gm * vm).sum())
+        print(f"[gpu]   dir {k}: fd {fd:.10e} adjoint {an:.10e} rel {abs(fd - an) / max(abs(an), 1e-30):.2e}",
+              flush=True)
+        fds.append(fd)
+        ans.append(an)
+    rms = float(np.sqrt(np.mean(np.square(ans))))
+    for fd, an in zip(fds, ans):
+        assert abs(fd - an) <= 1e-6 * max(abs(an), rms)
+
+
+def test_unit_weight_gradient_equals_fixed_order_gradient(require_gpu):
+    """With unit weights the mutual acceleration is the plain FMM's on the same
+    topology, so the two reverse passes agree to rounding."""
+    from jaccpot_amd import mutual as mu
+
+    n = 2000
+    x, m, topo, st = _build(n, theta=0.7, order=4, leaf=16, seed=15)
+    fmm_state = topo.prepared
+    w = np.random.default_rng(16).normal(size=(n, 3))
+    gx, gm = _mutual_loss_grad(mu, st, x, m, w, None, None)
+    from jaccpot_amd import FastMultipoleMethod
+
+    fmm = FastMultipoleMethod(precision="fp64", theta=0.7, softening=SOFT)
+    fmm_state.softening = SOFT
+    xt = torch.as_tensor(x, device="cuda").requires_grad_(True)
+    mt = torch.as_tensor(m, device="cuda").requires_grad_(True)
+    acc = fmm.differentiable_accelerations(fmm_state, xt, mt)
+    (acc * torch.as_tensor(w, device="cuda")).sum().backward()
+    ex = float(np.linalg.norm(gx - xt.grad.cpu().numpy()) / np.linalg.norm(gx))
+    em = float(np.linalg.norm(gm - mt.grad.cpu().numpy()) / np.linalg.norm(gm))
+    print(f"[gpu] mutual vs fixed-order gradient: positions {ex:.2e} masses {em:.2e}", flush=True)
+    assert ex < 1e-11 and em < 1e-11
