@@ -105,11 +105,12 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="plummer4m_p6", choices=sorted(WORKLOADS))
     ap.add_argument("--mode", default="learned",
-                    choices=("learned", "plain", "learned-only", "grad-only", "mutual"),
+                    choices=("learned", "plain", "learned-only", "grad-only", "mutual", "configs"),
                     help="learned: time the learned+mixed FMM (value) and the plain one beside it; "
                          "learned-only: the learned+mixed FMM alone (profiling); "
                          "grad-only: the differentiable forward + backward alone (profiling; prints its object); "
-                         "mutual: the momentum-conserving block-step lane (prints its own line)")
+                         "mutual: the momentum-conserving block-step lane (prints its own line); "
+                         "configs: one evaluation rate per benchmark configuration (prints its own line)")
     ap.add_argument("--mutual-leaf", type=int, default=8, help="leaf size of --mode mutual")
     ap.add_argument("--graph", action="store_true",
                     help="also time the step replayed from a captured HIP graph and report that as ms_per_step")
@@ -144,6 +145,9 @@ def main() -> int:
         return 0
     if args.mode == "mutual":
         print(json.dumps(measure_mutual(torch, ics, args)), flush=True)
+        return 0
+    if args.mode == "configs":
+        print(json.dumps(measure_configs(torch, FastMultipoleMethod, ics, args)), flush=True)
         return 0
 
     if args.calibrate_copy_mib > 0:
@@ -384,6 +388,72 @@ def main() -> int:
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def measure_configs(torch, FastMultipoleMethod, ics, args):
+    """One line per benchmark configuration: K evaluations at a frozen topology
+    (gather, upward, downward, L2P, P2P; ``step_device``, overlapped), inputs
+    resident on the GPU, and the force error on up to 4096 seeded targets
+    against the float64 direct sum.  uniform4k: N=4096 U[-1,1]^3, masses
+    U[0.5,1.5], float64, p 4, leaf 16, theta 0.6, no softening;
+    plummer1m: N=1,048,576 Plummer, float32, p 4, leaf 32, theta 0.6,
+    softening 1e-3; galaxy2m: N=2,097,152 (Hernquist halo 80% + exponential
+    disc 20%), float32, softening 1e-3, leaf 16, theta 0.7 -- the learned
+    gears (2, 4, 6) with float64 top levels (tolerance -2.5) and the plain
+    order-6 FMM; blobs1m: the differentiable forward + backward of
+    ``measure_gradient``."""
+    from jaccpot_amd import direct_sum_gravitational_acceleration
+
+    def run(tag, x, m, fmm, leaf, order, eps):
+        t0 = time.perf_counter()
+        st = fmm.prepare_state(x, m, leaf_size=leaf, max_order=order)
+        torch.cuda.synchronize()
+        prep = time.perf_counter() - t0
+        n = x.shape[0]
+        xd = torch.as_tensor(x, device="cuda").to(st.dtype)
+        md = torch.as_tensor(m, device="cuda").to(st.dtype)
+        acc = torch.empty((n, 3), dtype=st.dtype, device="cuda")
+        for _ in range(max(1, args.warmup)):
+            fmm.step_device(st, xd, md, acc, None)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(args.steps):
+            fmm.step_device(st, xd, md, acc, None)
+        b.record()
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / args.steps
+        tgt = np.sort(np.random.default_rng(12345).choice(n, size=min(4096, n), replace=False))
+        ref = direct_sum_gravitational_acceleration(x, m, softening=eps, target_indices=tgt)
+        got = acc[torch.as_tensor(tgt, device="cuda")].to(torch.float64)
+        rel = (torch.linalg.norm(got - ref, dim=1) / torch.linalg.norm(ref, dim=1)).cpu().numpy()
+        out = {"ms_per_step": ms, "particles_per_s": n / (ms * 1e-3), "prepare_s": prep,
+               "force_error": {"median": float(np.median(rel)), "p99": float(np.percentile(rel, 99)),
+                               "targets": int(tgt.size)}}
+        sched = st.extra.get("gears")
+        if sched is not None:
+            out["gear_counts"] = {str(k): v for k, v in sched.counts.items()}
+        log(f"[{tag}] {ms:.3f} ms, median err {out['force_error']['median']:.2e}")
+        del st, xd, md, acc
+        torch.cuda.empty_cache()
+        return out
+
+    res = {}
+    x, m = ics.uniform(4096, 0)
+    res["uniform4k_p4_fp64"] = run("uniform4k", x, m, FastMultipoleMethod(precision="fp64", theta=0.6), 16, 4, 0.0)
+    x, m = ics.plummer(1_048_576, 1)
+    res["plummer1m_p4_fp32"] = run("plummer1m", x.astype(np.float32), m.astype(np.float32),
+                                   FastMultipoleMethod(precision="fp32", theta=0.6, softening=1e-3), 32, 4, 1e-3)
+    x, m = ics.galaxy(2_097_152, 3)
+    x32, m32 = x.astype(np.float32), m.astype(np.float32)
+    res["galaxy2m_learned246_mixed"] = run(
+        "galaxy2m learned", x32, m32,
+        FastMultipoleMethod(precision="fp32", theta=0.7, softening=1e-3, adaptive_order=True, p_gears=(2, 4, 6),
+                            gear_tolerance=-2.5, mixed_precision=True), 16, 6, 1e-3)
+    res["galaxy2m_plain_p6"] = run("galaxy2m plain", x32, m32,
+                                   FastMultipoleMethod(precision="fp32", theta=0.7, softening=1e-3), 16, 6, 1e-3)
+    res["blobs1m_grad_p4_fp32"] = measure_gradient(torch, FastMultipoleMethod, ics, args)
+    return {"configs": res, "steps": args.steps, "warmup": args.warmup}
 
 
 def measure_mutual(torch, ics, args):
