@@ -471,6 +471,23 @@ int32_t jfm_mlp_to_bf16(void* stream, const float* master, uint16_t* wbf16);
 int32_t jfm_select_gears(void* stream, int32_t num_nodes, const int32_t* far_off, const int32_t* far_src,
                          const float* predictions, const void* centers4, float tol_log10, const int32_t* gears3,
                          int8_t* gears);
+/* Particle-level budget of the gear choice (gear_budget="leaf"):
+ * jfm_gear_node_max writes, per node, the largest top-gear error proxy of its
+ * far pairs (-inf without pairs); jfm_level_path_max (one tree level, root
+ * first) takes values[u] = max(values[u], values[parent[u]]);
+ * jfm_level_subtree_min (internal nodes of one level, deepest first) takes
+ * values[u] = min(values[left[u]], values[right[u]]); jfm_select_gears_budget
+ * is jfm_select_gears with those per-target budgets.  A refinement of the
+ * per-target rule for the reference's tagged gears
+ * (runtime/_adaptive_policy.py:2216-2385). */
+int32_t jfm_gear_node_max(void* stream, int32_t num_nodes, const int32_t* far_off, const int32_t* far_src,
+                          const float* predictions, const void* centers4, const int32_t* gears3, float* node_max);
+int32_t jfm_level_path_max(void* stream, const int32_t* nodes, int32_t count, const int32_t* parent, float* values);
+int32_t jfm_level_subtree_min(void* stream, const int32_t* nodes, int32_t count, const int32_t* left,
+                              const int32_t* right, float* values);
+int32_t jfm_select_gears_budget(void* stream, int32_t num_nodes, const int32_t* far_off, const int32_t* far_src,
+                                const float* predictions, const void* centers4, float tol_log10,
+                                const int32_t* gears3, const float* budget, int8_t* gears);
 
 /* Float64 direct sum at targets[nt] (tree slots) over all n particles of pos4
  * (float64), self excluded; writes acc [nt][3] and pot [nt] (pot may be NULL).

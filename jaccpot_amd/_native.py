@@ -28,7 +28,7 @@ MAX_ORDER = 8
 M2L_VALU = 0
 M2L_MFMA = 1
 M2L_ENGINES = {"valu": M2L_VALU, "mfma": M2L_MFMA}
-ABI_VERSION = 11
+ABI_VERSION = 12
 
 _i32p = POINTER(c_int32)
 _f64p = POINTER(c_double)
@@ -132,6 +132,12 @@ _SIGNATURES = {
     "jfm_center_adjoint_weighted": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_int32, c_void_p, c_void_p,
                                               c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
                                               c_void_p, c_void_p]),
+    "jfm_gear_node_max": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                    c_void_p]),
+    "jfm_level_path_max": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p, c_void_p]),
+    "jfm_level_subtree_min": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p]),
+    "jfm_select_gears_budget": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_float,
+                                          c_void_p, c_void_p, c_void_p]),
     "jfm_center_adjoint_gears": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_int32, c_void_p, c_void_p,
                                            c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                            c_void_p]),

@@ -504,6 +504,84 @@ __global__ __launch_bounds__(kThreads) void select_gears_kernel(int32_t num_node
   }
 }
 
+// Particle-level budget (gear_budget="leaf"): the per-node maximum top-gear
+// proxy (node_max_kernel), then along every root-to-leaf path the largest of
+// them (the dominant truncation term a particle of that leaf receives from
+// any level, path_max_kernel root first), then for every node the smallest of
+// its leaves' path maxima (subtree_min_kernel, deepest first): a pair of
+// target t may then err up to 10^tol of the dominant term of every particle
+// under t.
+__global__ __launch_bounds__(kThreads) void node_max_kernel(int32_t num_nodes, const int32_t* __restrict__ far_off,
+                                                            const int32_t* __restrict__ far_src,
+                                                            const float* __restrict__ Y,
+                                                            const float* __restrict__ centers4, int32_t g0,
+                                                            int32_t g1, int32_t g2, float* __restrict__ node_max) {
+  const int t = blockIdx.x * (kThreads / kWave) + static_cast<int>(threadIdx.x) / kWave;
+  const int lane = static_cast<int>(threadIdx.x) % kWave;
+  if (t >= num_nodes) return;  // wave-uniform
+  const int lo = far_off[t], hi = far_off[t + 1];
+  const V4<float> ct = load4(centers4, t);
+  const bool extrap = g2 >= 8;
+  const float slope = static_cast<float>(g2 - g1) / static_cast<float>(g1 - g0);
+  float m = -INFINITY;
+  for (int k = lo + lane; k < hi; k += kWave) {
+    const V4<float> cs4 = load4(centers4, far_src[k]);
+    const float dx = ct.x - cs4.x, dy = ct.y - cs4.y, dz = ct.z - cs4.z;
+    const float d2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-37f);
+    const float* y = Y + static_cast<int64_t>(k) * kOut;
+    const float top = extrap ? y[1] + slope * fminf(y[1] - y[0], 0.0f) : y[2];
+    m = fmaxf(m, top + log10f(fmaxf(cs4.w, 1e-37f) / d2));
+  }
+  for (int o = kWave / 2; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, kWave));
+  if (lane == 0) node_max[t] = m;
+}
+
+__global__ __launch_bounds__(kThreads) void path_max_kernel(const int32_t* __restrict__ nodes, int32_t count,
+                                                            const int32_t* __restrict__ parent,
+                                                            float* __restrict__ v) {
+  const int k = blockIdx.x * kThreads + threadIdx.x;
+  if (k >= count) return;
+  const int u = nodes[k];
+  v[u] = fmaxf(v[u], v[parent[u]]);
+}
+
+__global__ __launch_bounds__(kThreads) void subtree_min_kernel(const int32_t* __restrict__ nodes, int32_t count,
+                                                               const int32_t* __restrict__ left,
+                                                               const int32_t* __restrict__ right,
+                                                               float* __restrict__ v) {
+  const int k = blockIdx.x * kThreads + threadIdx.x;
+  if (k >= count) return;
+  const int u = nodes[k];
+  v[u] = fminf(v[left[u]], v[right[u]]);
+}
+
+// select_gears_kernel with the budget of every target given
+__global__ __launch_bounds__(kThreads) void select_budget_kernel(int32_t num_nodes, const int32_t* __restrict__ far_off,
+                                                                 const int32_t* __restrict__ far_src,
+                                                                 const float* __restrict__ Y,
+                                                                 const float* __restrict__ centers4, float tol,
+                                                                 int32_t g0, int32_t g1, int32_t g2,
+                                                                 const float* __restrict__ budget,
+                                                                 int8_t* __restrict__ gears) {
+  const int t = blockIdx.x * (kThreads / kWave) + static_cast<int>(threadIdx.x) / kWave;
+  const int lane = static_cast<int>(threadIdx.x) % kWave;
+  if (t >= num_nodes) return;
+  const int lo = far_off[t], hi = far_off[t + 1];
+  const V4<float> ct = load4(centers4, t);
+  const float b = budget[t];
+  for (int k = lo + lane; k < hi; k += kWave) {
+    const V4<float> cs4 = load4(centers4, far_src[k]);
+    const float dx = ct.x - cs4.x, dy = ct.y - cs4.y, dz = ct.z - cs4.z;
+    const float d2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-37f);
+    const float thr = b + tol - log10f(fmaxf(cs4.w, 1e-37f) / d2);
+    const float* y = Y + static_cast<int64_t>(k) * kOut;
+    int g = g2;
+    if (y[1] <= thr) g = g1;
+    if (y[0] <= thr) g = g0;
+    gears[k] = static_cast<int8_t>(g);
+  }
+}
+
 inline int nblocks(int64_t n, int b) { return static_cast<int>((n + b - 1) / b); }
 
 }  // namespace
@@ -614,6 +692,45 @@ int32_t jfm_select_gears(void* stream, int32_t num_nodes, const int32_t* far_off
       num_nodes, far_off, far_src, predictions, (const float*)centers4, tol_log10, gears3[0], gears3[1], gears3[2],
       gears);
   return check_launch("jfm_select_gears");
+}
+
+int32_t jfm_gear_node_max(void* stream, int32_t num_nodes, const int32_t* far_off, const int32_t* far_src,
+                          const float* predictions, const void* centers4, const int32_t* gears3, float* node_max) {
+  if (num_nodes < 1 || !far_off || !far_src || !predictions || !centers4 || !gears3 || !node_max)
+    return fail(JFM_ERR_ARG, "jfm_gear_node_max: bad arguments");
+  if (!(gears3[0] < gears3[1] && gears3[1] < gears3[2])) return fail(JFM_ERR_ARG, "jfm_gear_node_max: gears");
+  node_max_kernel<<<nblocks(num_nodes, kThreads / kWave), kThreads, 0, as_stream(stream)>>>(
+      num_nodes, far_off, far_src, predictions, (const float*)centers4, gears3[0], gears3[1], gears3[2], node_max);
+  return check_launch("jfm_gear_node_max");
+}
+
+int32_t jfm_level_path_max(void* stream, const int32_t* nodes, int32_t count, const int32_t* parent, float* values) {
+  if (count < 0) return fail(JFM_ERR_ARG, "jfm_level_path_max: negative count");
+  if (count == 0) return JFM_OK;
+  if (!nodes || !parent || !values) return fail(JFM_ERR_ARG, "jfm_level_path_max: NULL buffer");
+  path_max_kernel<<<nblocks(count, kThreads), kThreads, 0, as_stream(stream)>>>(nodes, count, parent, values);
+  return check_launch("jfm_level_path_max");
+}
+
+int32_t jfm_level_subtree_min(void* stream, const int32_t* nodes, int32_t count, const int32_t* left,
+                              const int32_t* right, float* values) {
+  if (count < 0) return fail(JFM_ERR_ARG, "jfm_level_subtree_min: negative count");
+  if (count == 0) return JFM_OK;
+  if (!nodes || !left || !right || !values) return fail(JFM_ERR_ARG, "jfm_level_subtree_min: NULL buffer");
+  subtree_min_kernel<<<nblocks(count, kThreads), kThreads, 0, as_stream(stream)>>>(nodes, count, left, right, values);
+  return check_launch("jfm_level_subtree_min");
+}
+
+int32_t jfm_select_gears_budget(void* stream, int32_t num_nodes, const int32_t* far_off, const int32_t* far_src,
+                                const float* predictions, const void* centers4, float tol_log10,
+                                const int32_t* gears3, const float* budget, int8_t* gears) {
+  if (num_nodes < 1 || !far_off || !far_src || !predictions || !centers4 || !gears3 || !budget || !gears)
+    return fail(JFM_ERR_ARG, "jfm_select_gears_budget: bad arguments");
+  if (!(gears3[0] < gears3[1] && gears3[1] < gears3[2])) return fail(JFM_ERR_ARG, "jfm_select_gears_budget: gears");
+  select_budget_kernel<<<nblocks(num_nodes, kThreads / kWave), kThreads, 0, as_stream(stream)>>>(
+      num_nodes, far_off, far_src, predictions, (const float*)centers4, tol_log10, gears3[0], gears3[1], gears3[2],
+      budget, gears);
+  return check_launch("jfm_select_gears_budget");
 }
 
 }  // extern "C"

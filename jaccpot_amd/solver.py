@@ -146,6 +146,7 @@ class FastMultipoleMethod:
         mixed_precision: bool = False,
         order_model: Any = None,
         gear_tolerance: float = -2.5,
+        gear_budget: str = "node",
         order_judge_targets: int = 1024,
         order_judge_factor: float = 2.0,
         order_judge_seed: int = 0,
@@ -220,6 +221,9 @@ class FastMultipoleMethod:
         self.mixed_precision = bool(mixed_precision)
         self.adaptive_order = bool(adaptive_order)
         self.gear_tolerance = float(gear_tolerance)
+        if gear_budget not in ("node", "leaf"):
+            raise ValueError(f"gear_budget must be 'node' or 'leaf', got {gear_budget!r}")
+        self.gear_budget = gear_budget
         self.order_judge_targets = int(order_judge_targets)
         self.order_judge_factor = float(order_judge_factor)
         self.order_judge_seed = int(order_judge_seed)
@@ -445,7 +449,7 @@ class FastMultipoleMethod:
         if K == 0:
             return
         X = pair_features(state)
-        g = self.order_model.select_gears(X, self.gear_tolerance, state)
+        g = self.order_model.select_gears(X, self.gear_tolerance, state, budget=self.gear_budget)
         state.extra["gears"] = build_gear_schedule(state, g, gears)
         state.extra["gear_of_pair"] = g.cpu().numpy()
 

@@ -265,6 +265,63 @@ def test_shipped_gear368_model_cuts_order8_work(require_gpu):
     assert el < e6
 
 
+def test_leaf_budget_never_raises_a_gear(require_gpu):
+    """gear_budget="leaf": a target's budget is the smallest, over its leaves,
+    of the largest top-gear proxy on the root-to-leaf path, which is never
+    below the target's own largest proxy (the "node" budget), so every pair's
+    gear is at most its node-budget gear; the budget arrays checked against
+    a NumPy pass over the same node maxima."""
+    from jaccpot_amd import FastMultipoleMethod
+    from jaccpot_amd import _native as nat
+    from jaccpot_amd.learned import pair_features
+    from jaccpot_amd.train_order import load, weights_path
+
+    print("[gpu] leaf gear budget vs node budget", flush=True)
+    x, m = ics.plummer(65_536, 17)
+    x32, m32 = x.astype(np.float32), m.astype(np.float32)
+    model = load(weights_path((3, 6, 8)))
+    fmm = FastMultipoleMethod(precision="fp32", theta=0.85, softening=1e-3)
+    st = fmm.prepare_state(x32, m32, leaf_size=8, max_order=8)
+    X = pair_features(st)
+    g_node = model.select_gears(X, -1.5, st, budget="node").cpu().numpy()
+    g_leaf = model.select_gears(X, -1.5, st, budget="leaf").cpu().numpy()
+    print(f"[gpu]   node {np.bincount(g_node, minlength=9)[[3, 6, 8]]} leaf {np.bincount(g_leaf, minlength=9)[[3, 6, 8]]}",
+          flush=True)
+    assert np.all(g_leaf <= g_node) and np.any(g_leaf < g_node)
+    # the budget pass against NumPy on the same node maxima
+    topo = st.topology
+    Y = model.predict(X)
+    g3 = np.asarray((3, 6, 8), np.int32)
+    b = torch.empty(topo.num_nodes, dtype=torch.float32, device="cuda")
+    nat.call("jfm_gear_node_max", nat.current_stream(), topo.num_nodes, st.far_off.data_ptr(), st.far_src.data_ptr(),
+             Y.data_ptr(), st.centers4.data_ptr(), nat.ptr(g3), b.data_ptr())
+    node_max = b.cpu().numpy().astype(np.float64)
+    path = node_max.copy()
+    for u in np.argsort(topo.depth, kind="stable"):  # root first
+        if topo.parent[u] >= 0:
+            path[u] = max(path[u], path[topo.parent[u]])
+    want = path.copy()
+    for u in np.argsort(-topo.depth[: topo.num_internal], kind="stable"):  # internal nodes, deepest first
+        want[u] = min(want[topo.left[u]], want[topo.right[u]])
+    off = topo.l2l_off
+    for lev in range(len(off) - 1):
+        a, e = int(off[lev]), int(off[lev + 1])
+        if e > a:
+            nat.call("jfm_level_path_max", nat.current_stream(), st.l2l_nodes.data_ptr() + 4 * a, e - a,
+                     st.parent.data_ptr(), b.data_ptr())
+    off = topo.m2m_off
+    for lev in range(len(off) - 1):
+        a, e = int(off[lev]), int(off[lev + 1])
+        if e > a:
+            nat.call("jfm_level_subtree_min", nat.current_stream(), st.m2m_nodes.data_ptr() + 4 * a, e - a,
+                     st.left.data_ptr(), st.right.data_ptr(), b.data_ptr())
+    got = b.cpu().numpy().astype(np.float64)
+    finite = np.isfinite(want)
+    assert np.array_equal(np.isfinite(got), finite)
+    assert np.array_equal(got[finite], want[finite])
+    assert np.all(got[finite] >= node_max[finite])
+
+
 def test_gear_schedule_matches_host_split(require_gpu):
     """The device gear split (radix sort by gear, CSR per gear, targets, flat
     lists and their spanning targets) equals a NumPy split of the far CSR."""
