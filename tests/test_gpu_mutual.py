@@ -97,25 +97,28 @@ def test_momentum_residual_beats_force_error(require_gpu):
     assert err > 1e-4 and mom < err * 1e-8
 
 
-@pytest.mark.parametrize("dtype, tol", [(np.float64, 1e-12), (np.float32, 1e-5)], ids=["fp64", "fp32"])
-def test_weighted_matches_cpu_restatement(dtype, tol, require_gpu):
-    """Rung-weighted accelerations and the near/far split against oracle/mutual.py."""
+@pytest.mark.parametrize("dtype, tol, leaf", [(np.float64, 1e-12, 16), (np.float32, 1e-5, 16), (np.float64, 1e-12, 12),
+                                             (np.float64, 1e-12, 24)], ids=["fp64", "fp32", "fp64_leaf12",
+                                                                             "fp64_leaf24"])
+def test_weighted_matches_cpu_restatement(dtype, tol, leaf, require_gpu):
+    """Rung-weighted accelerations and the near/far split against oracle/mutual.py
+    (leaves of 12 and 24 take the general slot decode)."""
     from jaccpot_amd import mutual as mu
 
     n = 3000
-    print(f"[gpu] mutual weighted vs restatement n={n} {np.dtype(dtype).name}", flush=True)
-    x, m, topo, st = _build(n, theta=0.8, order=5, leaf=16, seed=3, dtype=dtype)
+    print(f"[gpu] mutual weighted vs restatement n={n} {np.dtype(dtype).name} leaf {leaf}", flush=True)
+    x, m, topo, st = _build(n, theta=0.8, order=5, leaf=leaf, seed=3, dtype=dtype)
     rung = _rungs(n, 5)
     w = np.array([0.7, 0.3, 0.2, 0.05])
     got = mu.mutual_weighted_accelerations(st, x, m, rung=rung, level_weights=w, return_parts=True)
     x64, m64 = x.astype(np.float64), m.astype(np.float64)
-    acc, near, far = om.mutual_accelerations(x64, m64, leaf_size=16, p=5, theta=0.8, eps=SOFT, rung=rung,
+    acc, near, far = om.mutual_accelerations(x64, m64, leaf_size=leaf, p=5, theta=0.8, eps=SOFT, rung=rung,
                                              level_weights=w, return_parts=True)
     e = (_rel(got.accelerations, acc), _rel(got.near_forces, near), _rel(got.far_forces, far))
     print(f"[gpu]   rel acc {e[0]:.2e} near {e[1]:.2e} far {e[2]:.2e}", flush=True)
     assert max(e) < tol
     full = mu.mutual_accelerations(st, x, m)
-    ref = om.mutual_accelerations(x64, m64, leaf_size=16, p=5, theta=0.8, eps=SOFT)
+    ref = om.mutual_accelerations(x64, m64, leaf_size=leaf, p=5, theta=0.8, eps=SOFT)
     assert _rel(full, ref) < tol
 
 
