@@ -319,6 +319,9 @@ def main() -> int:
                 "frac": round(achieved / (PEAK_FP32_TFLOPS if precision == "fp32" else 78.6), 4),
                 "traffic": traffic, "traffic_source": traffic_note,
                 "interactions_per_launch": int(interactions), "flops_per_interaction": 20,
+                # the kernels are built without packed FP32 (two FMAs per lane per instruction), which is half of
+                # the published FP32 peak: against the unpacked FP32 VALU rate the same kernel reads
+                "frac_of_unpacked_fp32_valu": round(achieved / 78.6, 4) if precision == "fp32" else None,
                 "avg_launch_ms": round(head["p2p_alone_ms"], 4),
                 "measured": "P2P (p2p_kernel + split-leaf reduction) launched alone after the timed steps, HIP "
                             "events on its stream; inside the overlapped step it runs beside the tree passes"}
