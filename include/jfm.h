@@ -538,6 +538,20 @@ int32_t jfm_mutual_p2p(void* stream, int32_t dtype, const void* pos4, int32_t n,
                        int32_t leaf_size, const int32_t* near_off, const int32_t* near_src, const int32_t* rung_sorted,
                        const void* level_weights, int32_t num_levels, double eps2, double G, const int32_t* out_index,
                        void* force);
+/* jfm_mutual_p2p over work items that cut long near lists into bounded
+ * slot ranges (the reference chunks its near-pair work the same way,
+ * mutual/nearfield.py:58-83 resolve_near_chunk_size, there for memory).
+ * Items, partials and split arrays as jfm_p2p_items / jfm_p2p_reduce
+ * (whole source leaves per item; partials rows of 4, the fourth unused);
+ * the split leaves' partials are added in item order (compensated) and
+ * written to their force rows, so the result is deterministic. */
+int32_t jfm_mutual_p2p_items(void* stream, int32_t dtype, const void* pos4, int32_t n, int32_t num_leaves,
+                             int32_t leaf_size, int32_t num_items, const int32_t* item_leaf, const int32_t* item_s0,
+                             const int32_t* item_s1, const int32_t* item_part, int32_t num_part, void* partials,
+                             int32_t num_split, const int32_t* split_leaf, const int32_t* split_p0,
+                             const int32_t* split_p1, const int32_t* near_off, const int32_t* near_src,
+                             const int32_t* rung_sorted, const void* level_weights, int32_t num_levels, double eps2,
+                             double G, const int32_t* out_index, void* force);
 /* jfm_m2l_flat at gear p with pair (t <- s) scaled by
  * level_weights[clamp(max(node_rung[t], node_rung[s]), 0, num_levels - 1)]
  * (level_weights in device memory, the working dtype). */

@@ -28,7 +28,7 @@ MAX_ORDER = 8
 M2L_VALU = 0
 M2L_MFMA = 1
 M2L_ENGINES = {"valu": M2L_VALU, "mfma": M2L_MFMA}
-ABI_VERSION = 12
+ABI_VERSION = 13
 
 _i32p = POINTER(c_int32)
 _f64p = POINTER(c_double)
@@ -121,6 +121,10 @@ _SIGNATURES = {
     "jfm_mutual_rungs_level": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p]),
     "jfm_mutual_p2p": (c_int32, [c_void_p, c_int32, c_void_p, c_int32, c_int32, c_int32, c_void_p, c_void_p,
                                  c_void_p, c_void_p, c_int32, c_double, c_double, c_void_p, c_void_p]),
+    "jfm_mutual_p2p_items": (c_int32, [c_void_p, c_int32, c_void_p, c_int32, c_int32, c_int32, c_int32, c_void_p,
+                                       c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p,
+                                       c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_double, c_double,
+                                       c_void_p, c_void_p]),
     "jfm_m2l_flat_weighted": (c_int32, [c_void_p, c_int32, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
                                         c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_void_p,
                                         c_void_p, c_void_p]),

@@ -88,32 +88,60 @@ __device__ __forceinline__ void kahan_add(T& sum, T& comp, T x) {
   sum = t;
 }
 
-// One workgroup per target leaf.  Thread t holds target slot t mod WT
+// Work items over the source slots of the target leaves (whole source
+// leaves, a bounded number of slots each, so the few leaves with very long
+// near lists are spread over the GPU instead of running alone at the end of
+// the launch): item k covers slots [s0[k], s1[k]) of leaf leaf[k]'s sequence
+// (self leaf, then its near leaves, leaf_size slots each).  part[k] < 0: the
+// item is its leaf's only one and writes the force rows; otherwise it writes
+// its targets' partial sums to partials[(part[k] * leaf_size + slot) * 4 + c]
+// (c < 3; num_part rows of room) and mutual_reduce_kernel adds a split leaf's
+// partials in item order.  leaf == nullptr: one item per leaf over all of its
+// slots.
+template <typename T>
+struct MutualItems {
+  const int32_t* leaf;
+  const int32_t* s0;
+  const int32_t* s1;
+  const int32_t* part;
+  T* partials;
+  int32_t num_part;
+};
+
+// One workgroup per work item.  Thread t holds target slot t mod WT
 // (WT = leaf width rounded up to a power of two) and the source partition
-// t / WT; sources (self leaf, then the near leaves, leaf_size slots each)
-// stream through a kTile-entry LDS tile (kLoads loads per thread; the
-// next tile's positions and the near-list entries of the one after it are
-// fetched into registers during the current one), partition q taking the
-// tile entries q, q + NP, q + 2 NP, ... (NP = THREADS / WT partitions; the
-// partitions of a wave read neighbouring entries, so the LDS reads hit
-// distinct banks); the tile's memory then holds the partition sums.
-// Empty slots hold mass 0 at the origin and the self pair has d = 0, so
-// neither needs a branch: both add exactly zero.  Per-tile partials are
-// folded with Kahan compensation and the partitions are added in fixed order.
+// t / WT; the item's sources stream through a kTile-entry LDS tile, partition
+// q taking the tile entries q, q + NP, q + 2 NP, ... (NP = THREADS / WT
+// partitions; the partitions of a wave read neighbouring entries, so the LDS
+// reads hit distinct banks); the tile's memory then holds the partition sums.
+//
+// Fetch pipeline (kLoads slots per thread and tile): the near-list entries of
+// tile k + 2 and the positions (and rungs) of tile k + 1 are loaded while
+// tile k is in use.  Every load is unconditional, its index clamped into
+// range, and no loaded value is tested before the next tile boundary: a list
+// entry is decoded one tile after it is loaded, and a slot past the item's
+// end or past the last particle is flagged from values already in registers
+// and stored as a zero-mass source at the origin when the tile is written.
+// So the only wait for memory is at the tile store, one tile after the loads
+// were issued, and every partition runs the same wave-uniform trip count.
+// The self pair has d = 0 and a zero-mass source adds exactly zero, so
+// neither needs a branch.  Per-tile partials are folded with Kahan
+// compensation and the partitions are added in fixed order.
 template <typename T, bool WEIGHTED, int THREADS>
 __global__ __launch_bounds__(THREADS) void mutual_p2p_kernel(
     const T* __restrict__ pos4, int32_t n, int32_t leaf_size, int32_t wt_log2, int32_t leaf_log2,
     const int32_t* __restrict__ near_off, const int32_t* __restrict__ near_src,
     const int32_t* __restrict__ rung_sorted, const T* __restrict__ level_w, int32_t nlev, T eps2, T G,
-    const int32_t* __restrict__ out_index, T* __restrict__ force) {
-  constexpr int kLoads = 2;
+    const int32_t* __restrict__ out_index, T* __restrict__ force, MutualItems<T> items) {
+  constexpr int kLoads = sizeof(T) == 4 ? 4 : 2;  // 16 KiB of positions per tile either way
   constexpr int kTile = THREADS * kLoads;
   static_assert(kTile * 4 >= THREADS * 3, "the tile must hold the reduction");
   __shared__ V4<T> sp[kTile];
   __shared__ int32_t sr[WEIGHTED ? kTile : 1];
   __shared__ T lw[kMaxLevels];
   T(*red)[3] = reinterpret_cast<T(*)[3]>(sp);  // the tile, reused for the partition sums
-  const int leaf = xcd_contiguous(blockIdx.x, gridDim.x);  // neighbouring leaves share an XCD's L2
+  const int item = xcd_contiguous(blockIdx.x, gridDim.x);  // neighbouring leaves share an XCD's L2
+  const int leaf = items.leaf ? items.leaf[item] : item;
   const int t = threadIdx.x;
   const int wt = 1 << wt_log2;
   const int slot = t & (wt - 1);
@@ -131,54 +159,54 @@ __global__ __launch_bounds__(THREADS) void mutual_p2p_kernel(
     if (t < nlev) lw[t] = level_w[t];
   }
   const int b = near_off[leaf];
-  const int nsrc = (near_off[leaf + 1] - b + 1) * leaf_size;
-  T fx = T(0), fy = T(0), fz = T(0), cx = T(0), cy = T(0), cz = T(0);
-  // two-stage fetch while a tile is in use: the positions of the next tile
-  // (from source indices resolved one tile earlier) and the near-list
-  // entries of the tile after it
+  const int nnear = near_off[leaf + 1] - b;
+  const int start = items.leaf ? items.s0[item] : 0;
+  const int total = items.leaf ? items.s1[item] : (nnear + 1) * leaf_size;
+  const int part_out = items.leaf ? items.part[item] : -1;
+  const int32_t* list = nnear > 0 ? near_src + b : near_off;  // an empty list reads a valid address
+  const int hi = max(nnear - 1, 0);
+  auto seq_leaf = [&](int s) { return leaf_log2 >= 0 ? s >> leaf_log2 : s / leaf_size; };
+  int lst[kLoads];  // raw near-list entries, decoded one tile later
   V4<T> nx[kLoads];
+  bool ok[kLoads];
   int nr[kLoads];
-  int jl[kLoads];
-  auto resolve = [&](int base) {
+  auto list_entries = [&](int base) {
+#pragma unroll
+    for (int q = 0; q < kLoads; ++q) lst[q] = list[min(max(seq_leaf(base + q * THREADS + t) - 1, 0), hi)];
+  };
+  auto positions = [&](int base) {
 #pragma unroll
     for (int q = 0; q < kLoads; ++q) {
       const int s = base + q * THREADS + t;
-      int j = -1;
-      if (s < nsrc) {
-        const int ql = leaf_log2 >= 0 ? s >> leaf_log2 : s / leaf_size;
-        const int src_leaf = ql == 0 ? leaf : near_src[b + ql - 1];
-        const int jj = src_leaf * leaf_size + (s - ql * leaf_size);
-        if (jj < n) j = jj;
-      }
-      jl[q] = j;
+      const int ql = seq_leaf(s);
+      const int g = (ql == 0 ? leaf : lst[q]) * leaf_size + (s - ql * leaf_size);
+      ok[q] = s < total && g < n;
+      const int gi = ok[q] ? g : 0;
+      nx[q] = load4(pos4, gi);
+      if constexpr (WEIGHTED) nr[q] = rung_sorted[gi];
     }
   };
-  auto fetch = [&]() {
-#pragma unroll
-    for (int q = 0; q < kLoads; ++q) {
-      nx[q] = jl[q] >= 0 ? load4(pos4, jl[q]) : V4<T>{T(0), T(0), T(0), T(0)};
-      if constexpr (WEIGHTED) nr[q] = jl[q] >= 0 ? rung_sorted[jl[q]] : 0;
-    }
-  };
-  resolve(0);
-  fetch();
-  resolve(kTile);
-  for (int base = 0; base < nsrc; base += kTile) {
+  list_entries(start);
+  positions(start);
+  list_entries(start + kTile);
+  T fx = T(0), fy = T(0), fz = T(0), cx = T(0), cy = T(0), cz = T(0);
+  for (int base = start; base < total; base += kTile) {
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < kLoads; ++q) {
-      sp[q * THREADS + t] = nx[q];
+      sp[q * THREADS + t] = ok[q] ? nx[q] : V4<T>{T(0), T(0), T(0), T(0)};
       if constexpr (WEIGHTED) sr[q * THREADS + t] = nr[q];
     }
     __syncthreads();
-    if (base + kTile < nsrc) {
-      fetch();
-      resolve(base + 2 * kTile);
-    }
+    // unconditional (past the end they give flagged slots): a branch here
+    // would make the compiler wait for the loads at its join
+    positions(base + kTile);
+    list_entries(base + 2 * kTile);
+    const int trips = (min(kTile, total - base) + np - 1) / np;
     T px = T(0), py = T(0), pz = T(0);
-    const int lim = min(kTile, nsrc - base);
 #pragma unroll 4
-    for (int e = part; e < lim; e += np) {
+    for (int k = 0; k < trips; ++k) {
+      const int e = part + k * np;
       const V4<T> xj = sp[e];
       const T dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
       const T r2 = fma(dz, dz, fma(dy, dy, fma(dx, dx, eps2)));
@@ -207,11 +235,47 @@ __global__ __launch_bounds__(THREADS) void mutual_p2p_kernel(
       sy += red[q * wt + slot][1];
       sz += red[q * wt + slot][2];
     }
+    if (part_out >= 0) {
+      if (part_out < items.num_part) {
+        T* dst = items.partials + (static_cast<int64_t>(part_out) * leaf_size + slot) * 4;
+        dst[0] = sx;
+        dst[1] = sy;
+        dst[2] = sz;
+        dst[3] = T(0);
+      }
+      return;
+    }
     const int64_t r = out_index ? out_index[i] : i;
     force[3 * r + 0] = sx;
     force[3 * r + 1] = sy;
     force[3 * r + 2] = sz;
   }
+}
+
+// Split leaves: one thread per (split leaf, target slot) adds the leaf's item
+// partials p0[k] .. p1[k]-1 in item order (Kahan) and writes the force row.
+template <typename T>
+__global__ __launch_bounds__(256) void mutual_reduce_kernel(int32_t n, int32_t leaf_size, int32_t num_split,
+                                                            const int32_t* __restrict__ split_leaf,
+                                                            const int32_t* __restrict__ split_p0,
+                                                            const int32_t* __restrict__ split_p1,
+                                                            const T* __restrict__ partials,
+                                                            const int32_t* __restrict__ out_index, T* __restrict__ force) {
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g >= static_cast<int64_t>(num_split) * leaf_size) return;
+  const int k = static_cast<int>(g / leaf_size);
+  const int slot = static_cast<int>(g - static_cast<int64_t>(k) * leaf_size);
+  const int i = split_leaf[k] * leaf_size + slot;
+  if (i >= n) return;
+  T sum[3] = {T(0), T(0), T(0)}, comp[3] = {T(0), T(0), T(0)};
+  for (int p = split_p0[k]; p < split_p1[k]; ++p) {
+    const T* src = partials + (static_cast<int64_t>(p) * leaf_size + slot) * 4;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) kahan_add(sum[c], comp[c], src[c]);
+  }
+  const int64_t r = out_index ? out_index[i] : i;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) force[3 * r + c] = sum[c];
 }
 
 template <typename T>
@@ -247,10 +311,42 @@ struct MutualArgs {
 };
 
 template <typename T, bool W, int TH>
-void launch_mutual(const MutualArgs<T>& a, int num_leaves, hipStream_t s) {
-  mutual_p2p_kernel<T, W, TH><<<num_leaves, TH, 0, s>>>(a.pos4, a.n, a.leaf_size, a.wt_log2, a.leaf_log2,
+void launch_mutual(const MutualArgs<T>& a, const MutualItems<T>& items, int num_blocks, hipStream_t s) {
+  mutual_p2p_kernel<T, W, TH><<<num_blocks, TH, 0, s>>>(a.pos4, a.n, a.leaf_size, a.wt_log2, a.leaf_log2,
                                                         a.near_off, a.near_src, a.rung_sorted, a.level_w, a.nlev,
-                                                        a.eps2, a.G, a.out_index, a.force);
+                                                        a.eps2, a.G, a.out_index, a.force, items);
+}
+
+// Shared by the per-leaf and the work-item entry points: argument checks,
+// slot-decode constants, the near-field launch.
+int32_t mutual_near(const char* name, hipStream_t s, int32_t dtype, const void* pos4, int32_t n, int32_t num_leaves,
+                    int32_t leaf_size, int32_t num_blocks, const int32_t* item_leaf, const int32_t* item_s0,
+                    const int32_t* item_s1, const int32_t* item_part, int32_t num_part, void* partials,
+                    const int32_t* near_off, const int32_t* near_src, const int32_t* rung_sorted,
+                    const void* level_weights, int32_t num_levels, double eps2, double G, const int32_t* out_index,
+                    void* force) {
+  if (n < 1 || leaf_size < 1 || leaf_size > kMutualThreads || num_leaves != (n + leaf_size - 1) / leaf_size)
+    return fail(JFM_ERR_ARG, std::string(name) + ": inconsistent sizes (leaf_size must be 1..256)");
+  if (!pos4 || !near_off || !near_src || !force) return fail(JFM_ERR_ARG, std::string(name) + ": NULL buffer");
+  const bool weighted = level_weights != nullptr;
+  if (weighted && (!rung_sorted || num_levels < 1 || num_levels > kMaxLevels))
+    return fail(JFM_ERR_ARG, std::string(name) + ": weights need rungs and 1..64 levels");
+  int wt_log2 = 0;
+  while ((1 << wt_log2) < leaf_size) ++wt_log2;
+  const int leaf_log2 = (1 << wt_log2) == leaf_size ? wt_log2 : -1;
+  JFM_FOR_DTYPE(dtype, {
+    const T* lw = weighted ? (const T*)level_weights : nullptr;
+    const int32_t* rs = weighted ? rung_sorted : nullptr;
+    const int nl = weighted ? num_levels : 1;
+    const MutualArgs<T> a{(const T*)pos4, n, leaf_size, wt_log2, leaf_log2, near_off, near_src, rs, lw, nl,
+                          static_cast<T>(eps2), static_cast<T>(G), out_index, (T*)force};
+    const MutualItems<T> items{item_leaf, item_s0, item_s1, item_part, (T*)partials, num_part};
+    if (weighted)
+      launch_mutual<T, true, kMutualThreads>(a, items, num_blocks, s);
+    else
+      launch_mutual<T, false, kMutualThreads>(a, items, num_blocks, s);
+  });
+  return check_launch(name);
 }
 
 }  // namespace
@@ -283,28 +379,34 @@ int32_t jfm_mutual_p2p(void* stream, int32_t dtype, const void* pos4, int32_t n,
                        int32_t leaf_size, const int32_t* near_off, const int32_t* near_src, const int32_t* rung_sorted,
                        const void* level_weights, int32_t num_levels, double eps2, double G, const int32_t* out_index,
                        void* force) {
-  if (n < 1 || leaf_size < 1 || leaf_size > kMutualThreads || num_leaves != (n + leaf_size - 1) / leaf_size)
-    return fail(JFM_ERR_ARG, "jfm_mutual_p2p: inconsistent sizes (leaf_size must be 1..256)");
-  if (!pos4 || !near_off || !near_src || !force) return fail(JFM_ERR_ARG, "jfm_mutual_p2p: NULL buffer");
-  const bool weighted = level_weights != nullptr;
-  if (weighted && (!rung_sorted || num_levels < 1 || num_levels > kMaxLevels))
-    return fail(JFM_ERR_ARG, "jfm_mutual_p2p: weights need rungs and 1..64 levels");
-  int wt_log2 = 0;
-  while ((1 << wt_log2) < leaf_size) ++wt_log2;
-  const int leaf_log2 = (1 << wt_log2) == leaf_size ? wt_log2 : -1;
+  return mutual_near("jfm_mutual_p2p", as_stream(stream), dtype, pos4, n, num_leaves, leaf_size, num_leaves, nullptr,
+                     nullptr, nullptr, nullptr, 0, nullptr, near_off, near_src, rung_sorted, level_weights,
+                     num_levels, eps2, G, out_index, force);
+}
+
+int32_t jfm_mutual_p2p_items(void* stream, int32_t dtype, const void* pos4, int32_t n, int32_t num_leaves,
+                             int32_t leaf_size, int32_t num_items, const int32_t* item_leaf, const int32_t* item_s0,
+                             const int32_t* item_s1, const int32_t* item_part, int32_t num_part, void* partials,
+                             int32_t num_split, const int32_t* split_leaf, const int32_t* split_p0,
+                             const int32_t* split_p1, const int32_t* near_off, const int32_t* near_src,
+                             const int32_t* rung_sorted, const void* level_weights, int32_t num_levels, double eps2,
+                             double G, const int32_t* out_index, void* force) {
+  if (num_items < 1 || num_part < 0 || num_split < 0)
+    return fail(JFM_ERR_ARG, "jfm_mutual_p2p_items: need num_items >= 1, num_part >= 0, num_split >= 0");
+  if (!item_leaf || !item_s0 || !item_s1 || !item_part || (num_part > 0 && !partials) ||
+      (num_split > 0 && (!split_leaf || !split_p0 || !split_p1)))
+    return fail(JFM_ERR_ARG, "jfm_mutual_p2p_items: NULL item buffer");
   hipStream_t s = as_stream(stream);
+  const int32_t rc = mutual_near("jfm_mutual_p2p_items", s, dtype, pos4, n, num_leaves, leaf_size, num_items,
+                                 item_leaf, item_s0, item_s1, item_part, num_part, partials, near_off, near_src,
+                                 rung_sorted, level_weights, num_levels, eps2, G, out_index, force);
+  if (rc != JFM_OK || num_split == 0) return rc;
+  const int64_t rows = static_cast<int64_t>(num_split) * leaf_size;
   JFM_FOR_DTYPE(dtype, {
-    const T* lw = weighted ? (const T*)level_weights : nullptr;
-    const int32_t* rs = weighted ? rung_sorted : nullptr;
-    const int nl = weighted ? num_levels : 1;
-    const MutualArgs<T> a{(const T*)pos4, n, leaf_size, wt_log2, leaf_log2, near_off, near_src, rs, lw, nl,
-                          static_cast<T>(eps2), static_cast<T>(G), out_index, (T*)force};
-    if (weighted)
-      launch_mutual<T, true, kMutualThreads>(a, num_leaves, s);
-    else
-      launch_mutual<T, false, kMutualThreads>(a, num_leaves, s);
+    mutual_reduce_kernel<T><<<nblocks(rows, 256), 256, 0, s>>>(n, leaf_size, num_split, split_leaf, split_p0,
+                                                               split_p1, (const T*)partials, out_index, (T*)force);
   });
-  return check_launch("jfm_mutual_p2p");
+  return check_launch("jfm_mutual_p2p_items (reduce)");
 }
 
 int32_t jfm_mutual_combine(void* stream, int32_t dtype, int32_t n, const void* masses, const void* far_acc,

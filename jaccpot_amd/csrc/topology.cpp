@@ -129,7 +129,7 @@ struct WalkItem {
 
 extern "C" {
 
-int32_t jfm_abi_version(void) { return 12; }
+int32_t jfm_abi_version(void) { return 13; }
 
 const char* jfm_last_error(void) { return jfm::g_last_error.c_str(); }
 

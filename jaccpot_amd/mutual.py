@@ -11,7 +11,9 @@ topology.py, force.py) and of its block-step adapter
   (mutual/topology.py:300-395);
 * near field: ``jfm_mutual_p2p`` -- each pair force ``G m_i m_j w d / r^3``
   is evaluated from both ends with bit-negated results (no scatter, no
-  atomics), weighted by ``w[max(rung_i, rung_j)]``;
+  atomics), weighted by ``w[max(rung_i, rung_j)]``; long near lists are cut
+  into work items (``jfm_mutual_p2p_items``, the P2P's ``nearsplit`` items)
+  whose partial sums are added in item order;
 * far field: P2M, M2M, then the flat M2L over the symmetric far list with
   each pair scaled by ``w[max(rung_A, rung_B)]`` (cell rung = maximum over
   the cell's particles; ``jfm_m2l_flat_weighted``), L2L and L2P; the
@@ -220,6 +222,7 @@ class MutualFMMState:
     flat: Any = None
     level_off: np.ndarray = None  # m2m level offsets (deepest first) for the cell rungs
     buffers: dict = field(default_factory=dict)
+    near_items: Any = None  # work items over long near lists (nearsplit.P2PItems), None if every leaf fits one
 
     @property
     def num_particles(self) -> int:
@@ -241,6 +244,7 @@ def build_mutual_state(topology: MutualTopology, *, softening: float, G: float =
     ``near_chunk_size``, ``pallas_interpret`` and ``index_dtype`` select the
     reference's JAX lanes and change nothing here."""
     from .flatlist import build_flat_list, pair_targets
+    from .nearsplit import build_p2p_items
 
     del index_dtype
     prep = topology.prepared
@@ -270,7 +274,8 @@ def build_mutual_state(topology: MutualTopology, *, softening: float, G: float =
         inverse_permutation=d32(topology.inverse_permutation), softening=float(softening), G=float(G),
         order=int(topology.order), use_pallas=bool(use_pallas), near_chunk_size=near_chunk_size,
         pallas_interpret=bool(pallas_interpret), num_particles_=n, num_near_pairs=topology.num_near_pairs,
-        num_far_pairs=topology.num_far_pairs, flat=flat, level_off=np.asarray(prep.topology.m2m_off))
+        num_far_pairs=topology.num_far_pairs, flat=flat, level_off=np.asarray(prep.topology.m2m_off),
+        near_items=build_p2p_items(np.asarray(prep.topology.near_off), prep.topology.leaf_size, prep.dtype, dev))
     st.buffers = {
         "far_acc": torch.empty((n, 3), dtype=prep.dtype, device=dev),
         "rung_sorted": torch.empty(n, dtype=torch.int32, device=dev),
@@ -399,10 +404,19 @@ def _evaluate(st: MutualFMMState, x: torch.Tensor, m: torch.Tensor, lw):
              prep.centers4.data_ptr(), st.G, prep.perm.data_ptr(), far_acc.data_ptr(), None, 0)
     near = torch.empty((n, 3), dtype=prep.dtype, device=prep.device)
     weighted = lw is not None
-    nat.call("jfm_mutual_p2p", s, code, prep.pos4.data_ptr(), n, L, ls, prep.near_off.data_ptr(),
-             prep.near_src.data_ptr(), st.buffers["rung_sorted"].data_ptr() if weighted else None,
-             lw.data_ptr() if weighted else None, int(lw.numel()) if weighted else 0, st.softening ** 2, st.G,
-             prep.perm.data_ptr(), near.data_ptr())
+    rs = st.buffers["rung_sorted"].data_ptr() if weighted else None
+    lwp, nlev = (lw.data_ptr(), int(lw.numel())) if weighted else (None, 0)
+    it = st.near_items
+    if it is None:
+        nat.call("jfm_mutual_p2p", s, code, prep.pos4.data_ptr(), n, L, ls, prep.near_off.data_ptr(),
+                 prep.near_src.data_ptr(), rs, lwp, nlev, st.softening ** 2, st.G, prep.perm.data_ptr(),
+                 near.data_ptr())
+    else:
+        nat.call("jfm_mutual_p2p_items", s, code, prep.pos4.data_ptr(), n, L, ls, it.num_items, it.leaf.data_ptr(),
+                 it.s0.data_ptr(), it.s1.data_ptr(), it.part.data_ptr(), it.num_part, it.partials.data_ptr(),
+                 it.num_split, it.split_leaf.data_ptr(), it.split_p0.data_ptr(), it.split_p1.data_ptr(),
+                 prep.near_off.data_ptr(), prep.near_src.data_ptr(), rs, lwp, nlev, st.softening ** 2, st.G,
+                 prep.perm.data_ptr(), near.data_ptr())
     acc = torch.empty((n, 3), dtype=prep.dtype, device=prep.device)
     far = torch.empty((n, 3), dtype=prep.dtype, device=prep.device)
     nat.call("jfm_mutual_combine", s, code, n, m.data_ptr(), far_acc.data_ptr(), near.data_ptr(), acc.data_ptr(),

@@ -122,6 +122,48 @@ def test_weighted_matches_cpu_restatement(dtype, tol, leaf, require_gpu):
     assert _rel(full, ref) < tol
 
 
+@pytest.mark.parametrize("dtype, leaf", [(np.float64, 16), (np.float32, 16), (np.float64, 12)],
+                         ids=["f64", "f32", "f64_leaf12"])
+def test_near_work_items_match_per_leaf_launch_and_restatement(dtype, leaf, require_gpu):
+    """Near lists cut into work items of two source leaves, so every leaf with
+    a longer list is split and its partials go through the fixed-order
+    reduction: the near forces equal the one-workgroup-per-leaf launch to the
+    working precision's round-off, the accelerations match the restatement,
+    momentum stays at round-off, and repeated calls are bit-identical."""
+    from jaccpot_amd import mutual as mu
+    from jaccpot_amd.nearsplit import build_p2p_items
+
+    n = 3000
+    print(f"[gpu] mutual near work items n={n} {np.dtype(dtype).name} leaf {leaf}", flush=True)
+    x, m, topo, st = _build(n, theta=0.8, order=5, leaf=leaf, seed=3, dtype=dtype)
+    prep = st.prepared
+    assert st.near_items is None  # every near list fits one default item at this size
+    items = build_p2p_items(np.asarray(prep.topology.near_off), leaf, prep.dtype, prep.device, chunk_slots=2 * leaf)
+    assert items is not None and items.num_split > 0 and items.num_items > topo.num_leaves
+    rung = _rungs(n, 5)
+    w = np.array([0.7, 0.3, 0.2, 0.05])
+    per_leaf = mu.mutual_weighted_accelerations(st, x, m, rung=rung, level_weights=w, return_parts=True)
+    st.near_items = items
+    split = mu.mutual_weighted_accelerations(st, x, m, rung=rung, level_weights=w, return_parts=True)
+    again = mu.mutual_weighted_accelerations(st, x, m, rung=rung, level_weights=w, return_parts=True)
+    unit = mu.mutual_accelerations(st, x, m)
+    st.near_items = None
+    x64, m64 = x.astype(np.float64), m.astype(np.float64)
+    acc, near, _ = om.mutual_accelerations(x64, m64, leaf_size=leaf, p=5, theta=0.8, eps=SOFT, rung=rung,
+                                           level_weights=w, return_parts=True)
+    ref_unit = om.mutual_accelerations(x64, m64, leaf_size=leaf, p=5, theta=0.8, eps=SOFT)
+    f64 = dtype == np.float64
+    d_leaf = _rel(split.near_forces, np.asarray(per_leaf.near_forces, np.float64))
+    e = (_rel(split.accelerations, acc), _rel(split.near_forces, near), _rel(unit, ref_unit))
+    mom = om.momentum_residual(split.accelerations, m64)
+    print(f"[gpu]   items {items.num_items} split leaves {items.num_split}: vs per-leaf {d_leaf:.2e}, "
+          f"rel acc {e[0]:.2e} near {e[1]:.2e} unit {e[2]:.2e}, momentum {mom:.2e}", flush=True)
+    assert d_leaf < (1e-14 if f64 else 1e-6)
+    assert max(e) < (1e-12 if f64 else 1e-5)
+    assert mom < (1e-14 if f64 else 1e-6)
+    assert np.array_equal(split.near_forces, again.near_forces)
+
+
 def test_levels_partition_and_conserve_momentum(require_gpu):
     from jaccpot_amd import mutual as mu
 
