@@ -402,6 +402,15 @@ int32_t jfm_p2p_adjoint_weighted(void* stream, int32_t dtype, const void* pos4, 
                                  int32_t num_leaves, int32_t leaf_size, const int32_t* near_off,
                                  const int32_t* near_src, const int32_t* rung_sorted, const void* level_weights,
                                  int32_t num_levels, double eps2, double G, void* grad4);
+/* jfm_p2p_adjoint_weighted over work items (as jfm_p2p_adjoint_items; the
+ * split leaves' partials go through jfm_p2p_adjoint_reduce). */
+int32_t jfm_p2p_adjoint_weighted_items(void* stream, int32_t dtype, const void* pos4, const void* cot4, int32_t n,
+                                       int32_t num_leaves, int32_t leaf_size, int32_t num_items,
+                                       const int32_t* item_leaf, const int32_t* item_s0, const int32_t* item_s1,
+                                       const int32_t* item_part, int32_t num_part, void* partials,
+                                       const int32_t* near_off, const int32_t* near_src, const int32_t* rung_sorted,
+                                       const void* level_weights, int32_t num_levels, double eps2, double G,
+                                       void* grad4);
 int32_t jfm_center_adjoint_weighted(void* stream, int32_t dtype, int32_t p, const int32_t* nodes, int32_t num_nodes,
                                     const int32_t* far_off, const int32_t* far_src, const int32_t* node_rung,
                                     const void* level_weights, int32_t num_levels, const void* multipoles,
@@ -529,9 +538,10 @@ int32_t jfm_mutual_rungs(void* stream, int32_t n, int32_t num_leaves, int32_t le
                          const int32_t* rung, int32_t* rung_sorted, int32_t* node_rung);
 int32_t jfm_mutual_rungs_level(void* stream, const int32_t* nodes, int32_t count, const int32_t* left,
                                const int32_t* right, int32_t* node_rung);
-/* Near-field forces F_i = sum_j G m_i m_j w[max(r_i, r_j)] (x_j - x_i) /
+/* Near-field forces F_i = G sum_j m_i m_j w[max(r_i, r_j)] (x_j - x_i) /
  * (|x_j - x_i|^2 + eps2)^{3/2} over the self leaf (j != i) and the near
- * leaves, with f_ij == -f_ji to the last bit; rows force[out_index[i]][3]
+ * leaves; each pair term is the exact negative of its mirror (j, i) to the
+ * last bit, and G scales the per-target sums; rows force[out_index[i]][3]
  * (out_index NULL: tree order).  level_weights NULL: every pair weight 1
  * (rung_sorted unused).  leaf_size <= 256, num_levels <= 64. */
 int32_t jfm_mutual_p2p(void* stream, int32_t dtype, const void* pos4, int32_t n, int32_t num_leaves,

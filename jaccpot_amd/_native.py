@@ -133,6 +133,10 @@ _SIGNATURES = {
     "jfm_p2p_adjoint_weighted": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_int32, c_int32,
                                            c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_double, c_double,
                                            c_void_p]),
+    "jfm_p2p_adjoint_weighted_items": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_int32, c_int32,
+                                                 c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_void_p,
+                                                 c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_double, c_double,
+                                                 c_void_p]),
     "jfm_center_adjoint_weighted": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_int32, c_void_p, c_void_p,
                                               c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
                                               c_void_p, c_void_p]),

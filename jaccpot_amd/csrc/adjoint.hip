@@ -762,6 +762,36 @@ int32_t jfm_p2p_adjoint_weighted(void* stream, int32_t dtype, const void* pos4, 
   return fail(JFM_ERR_ARG, "jfm_p2p_adjoint_weighted: bad dtype");
 }
 
+int32_t jfm_p2p_adjoint_weighted_items(void* stream, int32_t dtype, const void* pos4, const void* cot4, int32_t n,
+                                       int32_t num_leaves, int32_t leaf_size, int32_t num_items,
+                                       const int32_t* item_leaf, const int32_t* item_s0, const int32_t* item_s1,
+                                       const int32_t* item_part, int32_t num_part, void* partials,
+                                       const int32_t* near_off, const int32_t* near_src, const int32_t* rung_sorted,
+                                       const void* level_weights, int32_t num_levels, double eps2, double G,
+                                       void* grad4) {
+  if (n < 1 || leaf_size < 1 || leaf_size > 256 || num_leaves != (n + leaf_size - 1) / leaf_size || num_items < 1 ||
+      num_part < 0)
+    return fail(JFM_ERR_ARG, "jfm_p2p_adjoint_weighted_items: inconsistent sizes");
+  if (num_levels < 1 || num_levels > kMaxLevelWeights)
+    return fail(JFM_ERR_ARG, "jfm_p2p_adjoint_weighted_items: 1..64 level weights");
+  if (!pos4 || !cot4 || !near_off || !near_src || !rung_sorted || !level_weights || !grad4 || !item_leaf ||
+      !item_s0 || !item_s1 || !item_part || (num_part > 0 && !partials))
+    return fail(JFM_ERR_ARG, "jfm_p2p_adjoint_weighted_items: NULL buffer");
+  hipStream_t s = as_stream(stream);
+  if (dtype == JFM_F32)
+    return launch_p2p_adj<float>(s, (const float*)pos4, (const float*)cot4, n, leaf_size, num_items, near_off,
+                                 near_src, static_cast<float>(eps2), static_cast<float>(G), (float*)grad4,
+                                 AdjItems<float>{item_leaf, item_s0, item_s1, item_part, (float*)partials, num_part},
+                                 PairLevels<float>{rung_sorted, (const float*)level_weights, num_levels});
+  if (dtype == JFM_F64)
+    return launch_p2p_adj<double>(s, (const double*)pos4, (const double*)cot4, n, leaf_size, num_items, near_off,
+                                  near_src, eps2, G, (double*)grad4,
+                                  AdjItems<double>{item_leaf, item_s0, item_s1, item_part, (double*)partials,
+                                                   num_part},
+                                  PairLevels<double>{rung_sorted, (const double*)level_weights, num_levels});
+  return fail(JFM_ERR_ARG, "jfm_p2p_adjoint_weighted_items: bad dtype");
+}
+
 int32_t jfm_center_adjoint_gears(void* stream, int32_t dtype, int32_t p, const int32_t* nodes, int32_t num_nodes,
                                  const int32_t* far_off, const int32_t* far_src, const int8_t* gear_fwd,
                                  const int8_t* gear_rev, const void* multipoles, const void* locals_own,

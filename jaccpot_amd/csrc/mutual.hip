@@ -8,7 +8,7 @@
 //  * near field: each target sums over its self leaf and near leaves with a
 //    pair force built so that f_ij == -f_ji to the last bit
 //      c = m_i * m_j (commutative), c *= w[max(rung_i, rung_j)],
-//      f = (c * inv_r^3) * G, F_i += f * (x_j - x_i),
+//      f = c * inv_r^3, F_i += f * (x_j - x_i), and F_i *= G once per target,
 //    where r^2 = fma(dz, dz, fma(dy, dy, fma(dx, dx, eps^2))) is the same
 //    for d and -d; the pair is evaluated from both ends, with bit-negated
 //    results, instead of being scattered (no atomics, deterministic);
@@ -63,21 +63,39 @@ __global__ __launch_bounds__(256) void node_rungs_kernel(const int32_t* __restri
   node_rung[u] = max(node_rung[left[u]], node_rung[right[u]]);
 }
 
-// 1/sqrt(r2), 0 for r2 <= 0.  float32: v_rsq_f32 (about 1 ulp); float64:
-// v_rsq_f64 refined by one third-order step of the Newton iteration for
-// r^-1/2, y <- y + y e (1/2 + 3 e / 8) with e = 1 - r2 y^2 (the relative
-// error is cubed).  A deterministic function of r2 alone, so the two ends of
-// a pair get the same value.
-template <typename T>
-__device__ __forceinline__ T inv_sqrt(T r2) {
+// r2^(-3/2); with SAFE, 0 for r2 <= 0 (no softening: the self pair and a
+// zero-mass slot at a target's own position).  float32: v_rsq_f32 (about
+// 1 ulp) cubed.  float64: the v_rsq_f64 estimate y refined in the cube
+// itself, r2^(-3/2) = y^3 (1 - e)^(-3/2) with e = 1 - r2 y^2, expanded to
+// y^3 (1 + 3e/2 + 15e^2/8) (the dropped terms are (35/16) e^3 relative).  A
+// deterministic function of r2 alone, so the two ends of a pair get the
+// same value.
+// k holds 15/8 in a vector register (see vgpr_const): an fma with two
+// constants that are not inline operands would need a copy per call.
+template <typename T, bool SAFE>
+__device__ __forceinline__ T inv_cube(T r2, T k) {
+  T z;
   if constexpr (sizeof(T) == 4) {
-    return r2 > T(0) ? __builtin_amdgcn_rsqf(r2) : T(0);
+    const T y = __builtin_amdgcn_rsqf(r2);
+    z = y * y * y;
   } else {
     const T y = __builtin_amdgcn_rsq(r2);
-    const T e = fma(-(r2 * y), y, T(1));
-    const T z = fma(y * e, fma(e, T(0.375), T(0.5)), y);
-    return r2 > T(0) ? z : T(0);
+    const T y2 = y * y;
+    const T e = fma(-r2, y2, T(1));
+    const T y3 = y2 * y;
+    z = fma(y3 * e, fma(e, k, T(1.5)), y3);
   }
+  if constexpr (SAFE) z = r2 > T(0) ? z : T(0);
+  return z;
+}
+
+// A loop-invariant constant kept in a vector register: the empty asm
+// statement hides the value from the compiler, which then neither folds it
+// into an instruction nor rematerialises it inside a loop.
+template <typename T>
+__device__ __forceinline__ T vgpr_const(T v) {
+  asm volatile("" : "+v"(v));
+  return v;
 }
 
 template <typename T>
@@ -126,8 +144,11 @@ struct MutualItems {
 // were issued, and every partition runs the same wave-uniform trip count.
 // The self pair has d = 0 and a zero-mass source adds exactly zero, so
 // neither needs a branch.  Per-tile partials are folded with Kahan
-// compensation and the partitions are added in fixed order.
-template <typename T, bool WEIGHTED, int THREADS>
+// compensation and the partitions are added in fixed order.  Sums are scaled
+// by G once per target (partials unscaled).  Rungs are clamped into
+// [0, nlev) as they are loaded, so a pair's level index is one max.
+// SAFE: guard r^2 == 0 (no softening).
+template <typename T, bool WEIGHTED, bool SAFE, int THREADS>
 __global__ __launch_bounds__(THREADS) void mutual_p2p_kernel(
     const T* __restrict__ pos4, int32_t n, int32_t leaf_size, int32_t wt_log2, int32_t leaf_log2,
     const int32_t* __restrict__ near_off, const int32_t* __restrict__ near_src,
@@ -153,7 +174,7 @@ __global__ __launch_bounds__(THREADS) void mutual_p2p_kernel(
   int ri = 0;
   if (live) {
     xi = load4(pos4, i);
-    if constexpr (WEIGHTED) ri = rung_sorted[i];
+    if constexpr (WEIGHTED) ri = min(max(rung_sorted[i], 0), nlev - 1);
   }
   if constexpr (WEIGHTED) {
     if (t < nlev) lw[t] = level_w[t];
@@ -189,13 +210,14 @@ __global__ __launch_bounds__(THREADS) void mutual_p2p_kernel(
   list_entries(start);
   positions(start);
   list_entries(start + kTile);
+  const T k158 = vgpr_const(T(1.875));
   T fx = T(0), fy = T(0), fz = T(0), cx = T(0), cy = T(0), cz = T(0);
   for (int base = start; base < total; base += kTile) {
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < kLoads; ++q) {
       sp[q * THREADS + t] = ok[q] ? nx[q] : V4<T>{T(0), T(0), T(0), T(0)};
-      if constexpr (WEIGHTED) sr[q * THREADS + t] = nr[q];
+      if constexpr (WEIGHTED) sr[q * THREADS + t] = min(max(nr[q], 0), nlev - 1);
     }
     __syncthreads();
     // unconditional (past the end they give flagged slots): a branch here
@@ -210,11 +232,9 @@ __global__ __launch_bounds__(THREADS) void mutual_p2p_kernel(
       const V4<T> xj = sp[e];
       const T dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
       const T r2 = fma(dz, dz, fma(dy, dy, fma(dx, dx, eps2)));
-      const T inv = inv_sqrt(r2);
-      const T inv3 = inv * inv * inv;
       T c = xi.w * xj.w;
-      if constexpr (WEIGHTED) c = c * lw[min(max(max(ri, sr[e]), 0), nlev - 1)];
-      const T f = (c * inv3) * G;
+      if constexpr (WEIGHTED) c = c * lw[max(ri, sr[e])];
+      const T f = c * inv_cube<T, SAFE>(r2, k158);
       px = fma(f, dx, px);
       py = fma(f, dy, py);
       pz = fma(f, dz, pz);
@@ -246,20 +266,20 @@ __global__ __launch_bounds__(THREADS) void mutual_p2p_kernel(
       return;
     }
     const int64_t r = out_index ? out_index[i] : i;
-    force[3 * r + 0] = sx;
-    force[3 * r + 1] = sy;
-    force[3 * r + 2] = sz;
+    force[3 * r + 0] = G * sx;
+    force[3 * r + 1] = G * sy;
+    force[3 * r + 2] = G * sz;
   }
 }
 
 // Split leaves: one thread per (split leaf, target slot) adds the leaf's item
-// partials p0[k] .. p1[k]-1 in item order (Kahan) and writes the force row.
+// partials p0[k] .. p1[k]-1 in item order (Kahan) and writes G times the sum.
 template <typename T>
 __global__ __launch_bounds__(256) void mutual_reduce_kernel(int32_t n, int32_t leaf_size, int32_t num_split,
                                                             const int32_t* __restrict__ split_leaf,
                                                             const int32_t* __restrict__ split_p0,
                                                             const int32_t* __restrict__ split_p1,
-                                                            const T* __restrict__ partials,
+                                                            const T* __restrict__ partials, T G,
                                                             const int32_t* __restrict__ out_index, T* __restrict__ force) {
   const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (g >= static_cast<int64_t>(num_split) * leaf_size) return;
@@ -275,7 +295,7 @@ __global__ __launch_bounds__(256) void mutual_reduce_kernel(int32_t n, int32_t l
   }
   const int64_t r = out_index ? out_index[i] : i;
 #pragma unroll
-  for (int c = 0; c < 3; ++c) force[3 * r + c] = sum[c];
+  for (int c = 0; c < 3; ++c) force[3 * r + c] = G * sum[c];
 }
 
 template <typename T>
@@ -310,9 +330,9 @@ struct MutualArgs {
   T* force;
 };
 
-template <typename T, bool W, int TH>
+template <typename T, bool W, bool SAFE, int TH>
 void launch_mutual(const MutualArgs<T>& a, const MutualItems<T>& items, int num_blocks, hipStream_t s) {
-  mutual_p2p_kernel<T, W, TH><<<num_blocks, TH, 0, s>>>(a.pos4, a.n, a.leaf_size, a.wt_log2, a.leaf_log2,
+  mutual_p2p_kernel<T, W, SAFE, TH><<<num_blocks, TH, 0, s>>>(a.pos4, a.n, a.leaf_size, a.wt_log2, a.leaf_log2,
                                                         a.near_off, a.near_src, a.rung_sorted, a.level_w, a.nlev,
                                                         a.eps2, a.G, a.out_index, a.force, items);
 }
@@ -341,10 +361,15 @@ int32_t mutual_near(const char* name, hipStream_t s, int32_t dtype, const void* 
     const MutualArgs<T> a{(const T*)pos4, n, leaf_size, wt_log2, leaf_log2, near_off, near_src, rs, lw, nl,
                           static_cast<T>(eps2), static_cast<T>(G), out_index, (T*)force};
     const MutualItems<T> items{item_leaf, item_s0, item_s1, item_part, (T*)partials, num_part};
-    if (weighted)
-      launch_mutual<T, true, kMutualThreads>(a, items, num_blocks, s);
+    const bool safe = !(a.eps2 > T(0));
+    if (weighted && safe)
+      launch_mutual<T, true, true, kMutualThreads>(a, items, num_blocks, s);
+    else if (weighted)
+      launch_mutual<T, true, false, kMutualThreads>(a, items, num_blocks, s);
+    else if (safe)
+      launch_mutual<T, false, true, kMutualThreads>(a, items, num_blocks, s);
     else
-      launch_mutual<T, false, kMutualThreads>(a, items, num_blocks, s);
+      launch_mutual<T, false, false, kMutualThreads>(a, items, num_blocks, s);
   });
   return check_launch(name);
 }
@@ -404,7 +429,8 @@ int32_t jfm_mutual_p2p_items(void* stream, int32_t dtype, const void* pos4, int3
   const int64_t rows = static_cast<int64_t>(num_split) * leaf_size;
   JFM_FOR_DTYPE(dtype, {
     mutual_reduce_kernel<T><<<nblocks(rows, 256), 256, 0, s>>>(n, leaf_size, num_split, split_leaf, split_p0,
-                                                               split_p1, (const T*)partials, out_index, (T*)force);
+                                                               split_p1, (const T*)partials, static_cast<T>(G),
+                                                               out_index, (T*)force);
   });
   return check_launch("jfm_mutual_p2p_items (reduce)");
 }

@@ -444,13 +444,26 @@ def _vjp(st: MutualFMMState, x: torch.Tensor, m: torch.Tensor, lw, acc_bar: torc
              cot4.data_ptr())
     grad4 = torch.empty((n, 4), dtype=dt, device=dev)
     eps2 = st.softening ** 2
-    if lw is not None:
+    it = st.near_items
+    if it is None and lw is not None:
         nat.call("jfm_p2p_adjoint_weighted", s, code, prep.pos4.data_ptr(), cot4.data_ptr(), n, L, ls,
                  prep.near_off.data_ptr(), prep.near_src.data_ptr(), st.buffers["rung_sorted"].data_ptr(),
                  lw.data_ptr(), int(lw.numel()), eps2, st.G, grad4.data_ptr())
-    else:
+    elif it is None:
         nat.call("jfm_p2p_adjoint", s, code, prep.pos4.data_ptr(), cot4.data_ptr(), n, L, ls,
                  prep.near_off.data_ptr(), prep.near_src.data_ptr(), eps2, st.G, grad4.data_ptr())
+    else:  # long near lists cut into the forward's work items; partials of its own
+        part = torch.empty_like(it.partials)
+        head = (s, code, prep.pos4.data_ptr(), cot4.data_ptr(), n, L, ls, it.num_items, it.leaf.data_ptr(),
+                it.s0.data_ptr(), it.s1.data_ptr(), it.part.data_ptr(), it.num_part, part.data_ptr(),
+                prep.near_off.data_ptr(), prep.near_src.data_ptr())
+        if lw is not None:
+            nat.call("jfm_p2p_adjoint_weighted_items", *head, st.buffers["rung_sorted"].data_ptr(), lw.data_ptr(),
+                     int(lw.numel()), eps2, st.G, grad4.data_ptr())
+        else:
+            nat.call("jfm_p2p_adjoint_items", *head, eps2, st.G, grad4.data_ptr())
+        nat.call("jfm_p2p_adjoint_reduce", s, code, cot4.data_ptr(), n, ls, it.num_split, it.split_leaf.data_ptr(),
+                 it.split_p0.data_ptr(), it.split_p1.data_ptr(), part.data_ptr(), eps2, st.G, grad4.data_ptr())
     lbar = torch.zeros_like(prep.locals)
     nat.call("jfm_leaf_adjoint_a", s, code, prep.order, prep.pos4.data_ptr(), cot4.data_ptr(), n, ls, I,
              prep.locals.data_ptr(), prep.centers4.data_ptr(), st.G, grad4.data_ptr(), lbar.data_ptr())

@@ -350,6 +350,33 @@ def test_gradient_matches_central_differences(weighted, require_gpu):
         assert abs(fd - an) <= 1e-6 * max(abs(an), rms)
 
 
+@pytest.mark.parametrize("weighted", [False, True], ids=["unit", "levels"])
+def test_gradient_through_near_work_items_equals_per_leaf_gradient(weighted, require_gpu):
+    """Near lists cut into work items of two source leaves in both passes (the
+    adjoint's split leaves go through the adjoint reduction): the gradient
+    equals the one-workgroup-per-leaf gradient to float64 round-off."""
+    from jaccpot_amd import mutual as mu
+    from jaccpot_amd.nearsplit import build_p2p_items
+
+    n = 1500
+    print(f"[gpu] mutual gradient through near work items ({'levels' if weighted else 'unit weights'})", flush=True)
+    x, m, topo, st = _build(n, theta=0.8, order=5, leaf=16, seed=12)
+    prep = st.prepared
+    items = build_p2p_items(np.asarray(prep.topology.near_off), 16, prep.dtype, prep.device, chunk_slots=32)
+    assert items is not None and items.num_split > 0
+    w = np.random.default_rng(13).normal(size=(n, 3))
+    rung = _rungs(n, 14) if weighted else None
+    lw = np.array([1.0, 0.5, 0.25, 0.125]) if weighted else None
+    gx0, gm0 = _mutual_loss_grad(mu, st, x, m, w, rung, lw)
+    st.near_items = items
+    gx1, gm1 = _mutual_loss_grad(mu, st, x, m, w, rung, lw)
+    st.near_items = None
+    ex, em = _rel(gx1, gx0), _rel(gm1, gm0)
+    print(f"[gpu]   items {items.num_items}, split leaves {items.num_split}: positions {ex:.2e} masses {em:.2e}",
+          flush=True)
+    assert ex < 1e-13 and em < 1e-13
+
+
 def test_unit_weight_gradient_equals_fixed_order_gradient(require_gpu):
     """With unit weights the mutual acceleration is the plain FMM's on the same
     topology, so the two reverse passes agree to rounding."""
