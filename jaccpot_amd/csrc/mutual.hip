@@ -138,17 +138,20 @@ struct MutualItems {
 // tile k is in use.  Every load is unconditional, its index clamped into
 // range, and no loaded value is tested before the next tile boundary: a list
 // entry is decoded one tile after it is loaded, and a slot past the item's
-// end or past the last particle is flagged from values already in registers
-// and stored as a zero-mass source at the origin when the tile is written.
+// end or past the last particle is flagged from values already in registers,
+// loads particle 0 and gets mass zero when the tile is written.
 // So the only wait for memory is at the tile store, one tile after the loads
 // were issued, and every partition runs the same wave-uniform trip count.
-// The self pair has d = 0 and a zero-mass source adds exactly zero, so
-// neither needs a branch.  Per-tile partials are folded with Kahan
+// The self pair has d = 0 and a zero-mass source adds exactly zero (r^2 > 0
+// with softening; without it SAFE maps r^2 == 0 to zero), so neither needs a
+// branch.  Per-tile partials are folded with Kahan
 // compensation and the partitions are added in fixed order.  Sums are scaled
 // by G once per target (partials unscaled).  Rungs are clamped into
 // [0, nlev) as they are loaded, so a pair's level index is one max.
-// SAFE: guard r^2 == 0 (no softening).
-template <typename T, bool WEIGHTED, bool SAFE, int THREADS>
+// SAFE: guard r^2 == 0 (no softening).  LEAF > 0: leaf_size == LEAF, a power
+// of two known at compile time (the partition stride then goes into the LDS
+// reads' immediate offsets); LEAF == 0: any leaf size, decoded at run time.
+template <typename T, bool WEIGHTED, bool SAFE, int LEAF, int THREADS>
 __global__ __launch_bounds__(THREADS) void mutual_p2p_kernel(
     const T* __restrict__ pos4, int32_t n, int32_t leaf_size, int32_t wt_log2, int32_t leaf_log2,
     const int32_t* __restrict__ near_off, const int32_t* __restrict__ near_src,
@@ -164,6 +167,12 @@ __global__ __launch_bounds__(THREADS) void mutual_p2p_kernel(
   const int item = xcd_contiguous(blockIdx.x, gridDim.x);  // neighbouring leaves share an XCD's L2
   const int leaf = items.leaf ? items.leaf[item] : item;
   const int t = threadIdx.x;
+  static_assert(LEAF == 0 || (LEAF & (LEAF - 1)) == 0, "compile-time leaves are powers of two");
+  if constexpr (LEAF > 0) {
+    wt_log2 = __builtin_ctz(LEAF);
+    leaf_log2 = wt_log2;
+    leaf_size = LEAF;
+  }
   const int wt = 1 << wt_log2;
   const int slot = t & (wt - 1);
   const int part = t >> wt_log2;
@@ -216,7 +225,9 @@ __global__ __launch_bounds__(THREADS) void mutual_p2p_kernel(
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < kLoads; ++q) {
-      sp[q * THREADS + t] = ok[q] ? nx[q] : V4<T>{T(0), T(0), T(0), T(0)};
+      V4<T> v = nx[q];
+      if (!ok[q]) v.w = T(0);
+      sp[q * THREADS + t] = v;
       if constexpr (WEIGHTED) sr[q * THREADS + t] = min(max(nr[q], 0), nlev - 1);
     }
     __syncthreads();
@@ -330,11 +341,27 @@ struct MutualArgs {
   T* force;
 };
 
-template <typename T, bool W, bool SAFE, int TH>
-void launch_mutual(const MutualArgs<T>& a, const MutualItems<T>& items, int num_blocks, hipStream_t s) {
-  mutual_p2p_kernel<T, W, SAFE, TH><<<num_blocks, TH, 0, s>>>(a.pos4, a.n, a.leaf_size, a.wt_log2, a.leaf_log2,
+template <typename T, bool W, bool SAFE, int LEAF, int TH>
+void launch_mutual_leaf(const MutualArgs<T>& a, const MutualItems<T>& items, int num_blocks, hipStream_t s) {
+  mutual_p2p_kernel<T, W, SAFE, LEAF, TH><<<num_blocks, TH, 0, s>>>(a.pos4, a.n, a.leaf_size, a.wt_log2, a.leaf_log2,
                                                         a.near_off, a.near_src, a.rung_sorted, a.level_w, a.nlev,
                                                         a.eps2, a.G, a.out_index, a.force, items);
+}
+
+template <typename T, bool W, bool SAFE, int TH>
+void launch_mutual(const MutualArgs<T>& a, const MutualItems<T>& items, int num_blocks, hipStream_t s) {
+  switch (a.leaf_size) {
+    case 8:
+      return launch_mutual_leaf<T, W, SAFE, 8, TH>(a, items, num_blocks, s);
+    case 16:
+      return launch_mutual_leaf<T, W, SAFE, 16, TH>(a, items, num_blocks, s);
+    case 32:
+      return launch_mutual_leaf<T, W, SAFE, 32, TH>(a, items, num_blocks, s);
+    case 64:
+      return launch_mutual_leaf<T, W, SAFE, 64, TH>(a, items, num_blocks, s);
+    default:
+      return launch_mutual_leaf<T, W, SAFE, 0, TH>(a, items, num_blocks, s);
+  }
 }
 
 // Shared by the per-leaf and the work-item entry points: argument checks,
