@@ -124,12 +124,19 @@ __device__ __forceinline__ void weighted_harmonics(T dx, T dy, T dz, T w, T bx, 
 // unscaled partial sums go to partials[(part * leaf_size + i) * 4 + c] and
 // p2p_adjoint_reduce_kernel adds them in item order.  leaf == NULL: one item
 // per leaf.
+// 1/sqrt(x) for x > 0 (x == 0 is mapped to 0 by the SAFE guard).  float32:
+// v_rsq_f32, as in the forward P2P; float64: the v_rsq_f64 estimate y with
+// one third-order step, y (1 + e/2 + 3e^2/8) with e = 1 - x y^2 (the relative
+// error is cubed).
 template <typename T>
 __device__ __forceinline__ T fast_rsqrt(T x) {
-  if constexpr (sizeof(T) == 4)
+  if constexpr (sizeof(T) == 4) {
     return __builtin_amdgcn_rsqf(x);
-  else
-    return T(1) / dev_sqrt(x);
+  } else {
+    const T y = __builtin_amdgcn_rsq(x);
+    const T e = fma(-(x * y), y, T(1));
+    return fma(y * e, fma(e, T(0.375), T(0.5)), y);
+  }
 }
 
 template <typename T>
@@ -209,8 +216,7 @@ __global__ __launch_bounds__(kThreads) void p2p_adjoint_kernel(const T* __restri
     __syncthreads();
     const int trips = (min(TILE, total - base) + NPART - 1) / NPART;
     // per pair: xbar += s3 u - (3 s5 (d.u) + s3 pw) d,  mbar -= s3 (d.abar_j) + s phib_j
-    // (the radial terms share one factor; s = 1/r from the hardware
-    // reciprocal square root in float32, as in the forward P2P)
+    // (the radial terms share one factor; s = 1/r from fast_rsqrt)
 #pragma unroll 4
     for (int k = 0; k < trips; ++k) {
       const V4<T> s = tx[part + k * NPART];
