@@ -27,6 +27,19 @@ constexpr int kWave = 64;
 // coefficients per pass of the LDS reductions: a 64 x 33 buffer (8.4 KiB for
 // float32) keeps LDS from limiting the waves resident per CU
 constexpr int kChunk = 32;
+// Coefficients per reduction pass.  The LDS buffer holds kWave rows of them,
+// and a workgroup's LDS decides how many one-wave workgroups share a CU, so
+// the C coefficients are split evenly into passes of at most kChunk (float32)
+// or kChunk / 2 (float64): float64 order 4 takes two passes of 13 through
+// 6.5 KiB instead of one pass through 17 KiB (3 -> 5 waves per SIMD), float32
+// order 3 one pass of 16 through 4.5 KiB.  Rows have an odd length (row_for),
+// so a lane's row starts in its own bank.
+constexpr int chunk_for(int c, int bytes) {
+  const int most = bytes == 4 ? kChunk : kChunk / 2;
+  const int passes = (c + most - 1) / most;
+  return (c + passes - 1) / passes;
+}
+constexpr int row_for(int c, int bytes) { return chunk_for(c, bytes) | 1; }
 
 // ---------------------------------------------------------------------------
 // Matrix-core form of the B_U applications (float32; m2l_engine "mfma").
@@ -173,7 +186,8 @@ __global__ __launch_bounds__(kWave) void m2l_kernel(const int32_t* __restrict__ 
                                                     const T* __restrict__ centers4, T* __restrict__ locals,
                                                     int32_t cs) {
   constexpr int C = ncoef(G);
-  __shared__ T red[kWave][kChunk + 1];
+  constexpr int CH = chunk_for(C, sizeof(T));
+  __shared__ T red[kWave][row_for(C, sizeof(T))];
   const int tgt = targets[xcd_contiguous(blockIdx.x, gridDim.x)];
   const int lane = threadIdx.x;
   BuRegs bu;
@@ -202,16 +216,16 @@ __global__ __launch_bounds__(kWave) void m2l_kernel(const int32_t* __restrict__ 
       for (int j = 0; j < C; ++j) acc[j] += w[j];
     }
   }
-  // lane partial sums reduced through LDS, kChunk coefficients at a time
+  // lane partial sums reduced through LDS, CH coefficients at a time
   T* row = locals + static_cast<int64_t>(tgt) * cs;
 #pragma unroll
-  for (int c0 = 0; c0 < C; c0 += kChunk) {
+  for (int c0 = 0; c0 < C; c0 += CH) {
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j)
+    for (int j = 0; j < CH; ++j)
       if (c0 + j < C) red[lane][j] = acc[c0 + j];
     __syncthreads();
-    if (lane < kChunk && c0 + lane < C) {
+    if (lane < CH && c0 + lane < C) {
       T s = T(0);
       for (int l = 0; l < kWave; ++l) s += red[l][lane];
       if (ACC)
@@ -252,7 +266,8 @@ __global__ __launch_bounds__(kWave) void m2l_flat_kernel(int32_t K, const int32_
                                                          T* __restrict__ locals, int32_t cs,
                                                          T* __restrict__ scratch, PairWeights<T> pw) {
   constexpr int C = ncoef(G);
-  __shared__ T red[kWave][kChunk + 1];
+  constexpr int CH = chunk_for(C, sizeof(T));
+  __shared__ T red[kWave][row_for(C, sizeof(T))];
   __shared__ int32_t tg[kWave];
   const int lane = threadIdx.x;
   BuRegs bu;
@@ -298,12 +313,12 @@ __global__ __launch_bounds__(kWave) void m2l_flat_kernel(int32_t K, const int32_
   const int before = k0 > 0 ? pair_tgt[k0 - 1] : -1;
   const int after = k0 + kWave < K ? pair_tgt[k0 + kWave] : -1;
   T* part = scratch + static_cast<int64_t>(blk) * 2 * C;
-  // the lanes' contributions go through LDS kChunk coefficients at a time
+  // the lanes' contributions go through LDS CH coefficients at a time
 #pragma unroll
-  for (int c0 = 0; c0 < C; c0 += kChunk) {
+  for (int c0 = 0; c0 < C; c0 += CH) {
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j)
+    for (int j = 0; j < CH; ++j)
       if (c0 + j < C) red[lane][j] = w[c0 + j];
     __syncthreads();
     const int j = c0 + lane;
@@ -313,7 +328,7 @@ __global__ __launch_bounds__(kWave) void m2l_flat_kernel(int32_t K, const int32_
       const int last = __builtin_ctzll(ends);
       ends &= ends - 1;
       const int tl = __builtin_amdgcn_readfirstlane(tg[last]);  // the same for every lane
-      if (tl >= 0 && lane < kChunk && j < C) {
+      if (tl >= 0 && lane < CH && j < C) {
         const bool cont_before = start == 0 && before == tl;
         const bool cont_after = last == kWave - 1 && after == tl;
         T s = T(0);
