@@ -164,6 +164,36 @@ def test_near_work_items_match_per_leaf_launch_and_restatement(dtype, leaf, requ
     assert np.array_equal(split.near_forces, again.near_forces)
 
 
+def test_default_state_splits_long_near_lists(require_gpu):
+    """A truncated Hernquist sphere (N=20000, leaf 16): its sparse outer leaves
+    see every particle, so build_mutual_state itself cuts those near lists
+    into work items; the result equals the one-workgroup-per-leaf launch to
+    round-off, with momentum at round-off, unit and level weights."""
+    from jaccpot_amd import ics
+    from jaccpot_amd import mutual as mu
+
+    n = 20000
+    print(f"[gpu] mutual default work items, Hernquist n={n} leaf 16", flush=True)
+    x, m = ics.galaxy(n, 3, halo_fraction=1.0)
+    topo, _ = mu.build_mutual_topology(x, m, theta=0.7, order=4, leaf_size=16)
+    st = mu.build_mutual_state(topo, softening=1e-3)
+    items = st.near_items
+    assert items is not None and items.num_split > 0
+    rung = _rungs(n, 7)
+    w = np.array([1.0, 0.5, 0.25, 0.125])
+    split = mu.mutual_accelerations(st, x, m)
+    split_w = mu.mutual_weighted_accelerations(st, x, m, rung=rung, level_weights=w)
+    st.near_items = None
+    per_leaf = mu.mutual_accelerations(st, x, m)
+    per_leaf_w = mu.mutual_weighted_accelerations(st, x, m, rung=rung, level_weights=w)
+    st.near_items = items
+    d, dw = _rel(split, per_leaf), _rel(split_w, per_leaf_w)
+    mom = om.momentum_residual(split, m)
+    print(f"[gpu]   items {items.num_items} split leaves {items.num_split}: vs per-leaf {d:.2e} / {dw:.2e}, "
+          f"momentum {mom:.2e}", flush=True)
+    assert d < 1e-14 and dw < 1e-14 and mom < 1e-14
+
+
 def test_levels_partition_and_conserve_momentum(require_gpu):
     from jaccpot_amd import mutual as mu
 
