@@ -345,39 +345,45 @@ __global__ __launch_bounds__(kWave) void m2l_flat_kernel(int32_t K, const int32_
   }
 }
 
-// One wave per target whose pairs span waves w0 < w1 of the flat launch (four
-// targets per workgroup): local += partial(w0, slot 1) + sum_{w0 < w <= w1} partial(w, slot 0).
+// Targets whose pairs span waves w0 < w1 of the flat launch:
+// local += partial(w0, slot 1) + sum_{w0 < w <= w1} partial(w, slot 0), one
+// thread per (target, coefficient), so no lane idles on a coefficient count
+// that is not a multiple of the wave and every thread's loads are in flight
+// together.  Block partials are added in block order; the low gears, whose
+// targets span many blocks, load eight at a time (one memory round trip per
+// eight blocks).
+constexpr int kFixupThreads = 256;
+inline int fixup_blocks(int32_t num_span, int c) {
+  return static_cast<int>((static_cast<int64_t>(num_span) * c + kFixupThreads - 1) / kFixupThreads);
+}
+
 template <typename T, int G>
-__global__ __launch_bounds__(4 * kWave) void m2l_fixup_kernel(int32_t num_span, const int32_t* __restrict__ span_tgt,
-                                                              const int32_t* __restrict__ span_w0,
-                                                              const int32_t* __restrict__ span_w1,
-                                                              const T* __restrict__ scratch, T* __restrict__ locals,
-                                                              int32_t cs) {
+__global__ __launch_bounds__(kFixupThreads) void m2l_fixup_kernel(int32_t num_span,
+                                                                  const int32_t* __restrict__ span_tgt,
+                                                                  const int32_t* __restrict__ span_w0,
+                                                                  const int32_t* __restrict__ span_w1,
+                                                                  const T* __restrict__ scratch,
+                                                                  T* __restrict__ locals, int32_t cs) {
   constexpr int C = ncoef(G);
-  const int sp = blockIdx.x * 4 + static_cast<int>(threadIdx.x) / kWave;
-  if (sp >= num_span) return;
-  const int lane = threadIdx.x % kWave;
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * kFixupThreads + threadIdx.x;
+  if (g >= static_cast<int64_t>(num_span) * C) return;
+  const int sp = static_cast<int>(g / C);
+  const int j = static_cast<int>(g - static_cast<int64_t>(sp) * C);
   const int t = span_tgt[sp];
   const int w0 = span_w0[sp], w1 = span_w1[sp];
-  for (int j = lane; j < C; j += kWave) {
-    T s = scratch[static_cast<int64_t>(w0) * 2 * C + C + j];
-    // block partials added in block order; for rows of one wave pass (the
-    // low gears, whose targets span many blocks) loaded eight at a time, one
-    // memory round trip per eight blocks (the order-8 spans are short and
-    // the unrolled loop costs that kernel occupancy)
-    int w = w0 + 1;
-    if constexpr (C <= kWave) {
-      for (; w + 7 <= w1; w += 8) {
-        T v[8];
+  T s = scratch[static_cast<int64_t>(w0) * 2 * C + C + j];
+  int w = w0 + 1;
+  if constexpr (C <= kWave) {
+    for (; w + 7 <= w1; w += 8) {
+      T v[8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) v[q] = scratch[static_cast<int64_t>(w + q) * 2 * C + j];
+      for (int q = 0; q < 8; ++q) v[q] = scratch[static_cast<int64_t>(w + q) * 2 * C + j];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) s += v[q];
-      }
+      for (int q = 0; q < 8; ++q) s += v[q];
     }
-    for (; w <= w1; ++w) s += scratch[static_cast<int64_t>(w) * 2 * C + j];
-    locals[static_cast<int64_t>(t) * cs + j] += s;
   }
+  for (; w <= w1; ++w) s += scratch[static_cast<int64_t>(w) * 2 * C + j];
+  locals[static_cast<int64_t>(t) * cs + j] += s;
 }
 
 }  // namespace
@@ -504,8 +510,8 @@ int32_t launch_flat(hipStream_t s, int32_t gear, int32_t K, const int32_t* pair_
     if constexpr (G <= P) {                                                                                      \
       launch_flat_pairs<T, P, G>(s, nw, K, pair_tgt, pair_src, mult, centers4, locals, scratch);                 \
       if (num_span > 0)                                                                                          \
-        m2l_fixup_kernel<T, G><<<(num_span + 3) / 4, 4 * kWave, 0, s>>>(num_span, span_tgt, span_w0, span_w1,   \
-                                                                        (const T*)scratch, (T*)locals, cs);      \
+        m2l_fixup_kernel<T, G><<<fixup_blocks(num_span, ncoef(G)), kFixupThreads, 0, s>>>(                      \
+            num_span, span_tgt, span_w0, span_w1, (const T*)scratch, (T*)locals, cs);                           \
       return check_launch("jfm_m2l_flat");                                                                       \
     }                                                                                                            \
   }
@@ -553,8 +559,8 @@ extern "C" int32_t jfm_m2l_flat_weighted(void* stream, int32_t dtype, int32_t p,
     m2l_flat_kernel<T, P, P, false, true><<<nw, kWave, 0, s>>>(num_pairs, pair_tgt, pair_src, (const T*)multipoles,
                                                                (const T*)centers4, (T*)locals, cs, (T*)scratch, pw);
     if (num_span > 0)
-      m2l_fixup_kernel<T, P><<<(num_span + 3) / 4, 4 * kWave, 0, s>>>(num_span, span_tgt, span_w0, span_w1,
-                                                                     (const T*)scratch, (T*)locals, cs);
+      m2l_fixup_kernel<T, P><<<fixup_blocks(num_span, ncoef(P)), kFixupThreads, 0, s>>>(
+          num_span, span_tgt, span_w0, span_w1, (const T*)scratch, (T*)locals, cs);
     return check_launch("jfm_m2l_flat_weighted");
   }));
   return JFM_OK;
