@@ -346,15 +346,21 @@ __global__ __launch_bounds__(kWave) void m2l_flat_kernel(int32_t K, const int32_
 }
 
 // Targets whose pairs span waves w0 < w1 of the flat launch:
-// local += partial(w0, slot 1) + sum_{w0 < w <= w1} partial(w, slot 0), one
-// thread per (target, coefficient), so no lane idles on a coefficient count
-// that is not a multiple of the wave and every thread's loads are in flight
-// together.  Block partials are added in block order; the low gears, whose
-// targets span many blocks, load eight at a time (one memory round trip per
-// eight blocks).
+// local += partial(w0, slot 1) + sum_{w0 < w <= w1} partial(w, slot 0).
+// Above 16 coefficients each thread takes four (target, coefficient)
+// entries, a workgroup's entries kFixupThreads apart so that every load stays
+// coalesced over the coefficients.  The loads a sum needs whatever the span
+// (its span record, slot 1 of w0, slot 0 of w0 + 1 and the local) are issued
+// for all the thread's entries before the first add: a span of two waves, the
+// usual one at the high gears, costs two memory round trips instead of four.
+// The low gears, whose targets span many blocks, keep one entry per thread
+// and load eight blocks at a time (one memory round trip per eight blocks).
+// Block partials are added in block order.
 constexpr int kFixupThreads = 256;
+constexpr int fixup_items(int c) { return c <= 16 ? 1 : 4; }
 inline int fixup_blocks(int32_t num_span, int c) {
-  return static_cast<int>((static_cast<int64_t>(num_span) * c + kFixupThreads - 1) / kFixupThreads);
+  const int64_t per = int64_t(kFixupThreads) * fixup_items(c);
+  return static_cast<int>((static_cast<int64_t>(num_span) * c + per - 1) / per);
 }
 
 template <typename T, int G>
@@ -365,25 +371,46 @@ __global__ __launch_bounds__(kFixupThreads) void m2l_fixup_kernel(int32_t num_sp
                                                                   const T* __restrict__ scratch,
                                                                   T* __restrict__ locals, int32_t cs) {
   constexpr int C = ncoef(G);
-  const int64_t g = static_cast<int64_t>(blockIdx.x) * kFixupThreads + threadIdx.x;
-  if (g >= static_cast<int64_t>(num_span) * C) return;
-  const int sp = static_cast<int>(g / C);
-  const int j = static_cast<int>(g - static_cast<int64_t>(sp) * C);
-  const int t = span_tgt[sp];
-  const int w0 = span_w0[sp], w1 = span_w1[sp];
-  T s = scratch[static_cast<int64_t>(w0) * 2 * C + C + j];
-  int w = w0 + 1;
-  if constexpr (C <= kWave) {
-    for (; w + 7 <= w1; w += 8) {
-      T v[8];
+  constexpr int Q = fixup_items(C);
+  const int64_t total = static_cast<int64_t>(num_span) * C;
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * (kFixupThreads * Q) + threadIdx.x;
+  bool live[Q];
+  int j[Q], w0[Q], w1[Q];
+  int64_t at[Q];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) v[q] = scratch[static_cast<int64_t>(w + q) * 2 * C + j];
-#pragma unroll
-      for (int q = 0; q < 8; ++q) s += v[q];
-    }
+  for (int q = 0; q < Q; ++q) {
+    const int64_t g = base + static_cast<int64_t>(q) * kFixupThreads;
+    live[q] = g < total;
+    const int sp = live[q] ? static_cast<int>(g / C) : 0;  // dead entries read span 0 and store nothing
+    j[q] = live[q] ? static_cast<int>(g - static_cast<int64_t>(sp) * C) : 0;
+    at[q] = static_cast<int64_t>(span_tgt[sp]) * cs + j[q];
+    w0[q] = span_w0[sp];
+    w1[q] = span_w1[sp];
   }
-  for (; w <= w1; ++w) s += scratch[static_cast<int64_t>(w) * 2 * C + j];
-  locals[static_cast<int64_t>(t) * cs + j] += s;
+  T s[Q], nx[Q], loc[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    s[q] = scratch[static_cast<int64_t>(w0[q]) * 2 * C + C + j[q]];
+    nx[q] = scratch[static_cast<int64_t>(w0[q] + 1) * 2 * C + j[q]];  // w1 > w0 for every span
+    loc[q] = locals[at[q]];
+  }
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    T v = s[q] + nx[q];
+    int w = w0[q] + 2;
+    const int last = w1[q];
+    if constexpr (C <= kWave) {
+      for (; w + 7 <= last; w += 8) {
+        T b[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) b[r] = scratch[static_cast<int64_t>(w + r) * 2 * C + j[q]];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) v += b[r];
+      }
+    }
+    for (; w <= last; ++w) v += scratch[static_cast<int64_t>(w) * 2 * C + j[q]];
+    if (live[q]) locals[at[q]] = loc[q] + v;
+  }
 }
 
 }  // namespace

@@ -311,7 +311,10 @@ __global__ __launch_bounds__(256) void l2p_combine_kernel(int64_t n3, int32_t n,
 // Mixed precision: the internal nodes of the top three levels (depth <= 2,
 // at most 7 nodes) keep float64 shadow rows.  One workgroup walks their M2M
 // levels deepest first in float64, reading float32 rows for the children
-// below, and writes both the shadow and the rounded float32 row.
+// below, and writes both the shadow and the rounded float32 row.  Two lanes
+// per node as in m2m_kernel: lane 2i translates the left child, lane 2i+1 the
+// right one, and the even lane writes left + right, so a level waits for one
+// child's translation instead of two.
 template <int P>
 __global__ __launch_bounds__(kWave) void m2m_top_kernel(const int32_t* __restrict__ nodes,
                                                         const int32_t* __restrict__ lev_off, int32_t nlev,
@@ -323,59 +326,72 @@ __global__ __launch_bounds__(kWave) void m2m_top_kernel(const int32_t* __restric
                                                         double* __restrict__ mult64, int32_t cs64,
                                                         double* __restrict__ centers64) {
   constexpr int C = ncoef(P);
+  const int t = threadIdx.x;
+  const int side = t & 1;
   for (int lev = 0; lev < nlev; ++lev) {
-    const int t = threadIdx.x;
     const int count = lev_off[lev + 1] - lev_off[lev];
-    if (t < count) {
-      const int node = nodes[lev_off[lev] + t];
-      const int ch[2] = {left[node], right[node]};
-      V4<double> cc[2];
-      double vc[2][C];
-      for (int side = 0; side < 2; ++side) {
-        const int c = ch[side];
-        const int sh = shadow[c];
-        if (sh >= 0) {
-          cc[side] = load4(centers64, sh);
-          load_row<double, C>(mult64 + static_cast<int64_t>(sh) * cs64, vc[side]);
+    if ((t >> 1) < count) {  // both lanes of a pair take this branch together
+      const int node = nodes[lev_off[lev] + (t >> 1)];
+      const int lc = left[node], rc = right[node];
+      V4<double> cl, cr;
+      {
+        const int sl = shadow[lc], sr = shadow[rc];
+        if (sl >= 0) {
+          cl = load4(centers64, sl);
         } else {
-          const V4<float> f = load4(centers4, c);
-          cc[side] = V4<double>{f.x, f.y, f.z, f.w};
-          float tmp[C];
-          load_row<float, C>(mult + static_cast<int64_t>(c) * cs, tmp);
-          for (int j = 0; j < C; ++j) vc[side][j] = tmp[j];
+          const V4<float> f = load4(centers4, lc);
+          cl = V4<double>{f.x, f.y, f.z, f.w};
+        }
+        if (sr >= 0) {
+          cr = load4(centers64, sr);
+        } else {
+          const V4<float> f = load4(centers4, rc);
+          cr = V4<double>{f.x, f.y, f.z, f.w};
         }
       }
-      const double tot = cc[0].w + cc[1].w;
+      const double tot = cl.w + cr.w;
       double cx, cy, cz;
       if (tot > 0.0) {
-        cx = (cc[0].w * cc[0].x + cc[1].w * cc[1].x) / tot;
-        cy = (cc[0].w * cc[0].y + cc[1].w * cc[1].y) / tot;
-        cz = (cc[0].w * cc[0].z + cc[1].w * cc[1].z) / tot;
+        cx = (cl.w * cl.x + cr.w * cr.x) / tot;
+        cy = (cl.w * cl.y + cr.w * cr.y) / tot;
+        cz = (cl.w * cl.z + cr.w * cr.z) / tot;
       } else {
-        const double nl = node_range[2 * ch[0] + 1] - node_range[2 * ch[0]] + 1;
-        const double nr = node_range[2 * ch[1] + 1] - node_range[2 * ch[1]] + 1;
-        cx = (nl * cc[0].x + nr * cc[1].x) / (nl + nr);
-        cy = (nl * cc[0].y + nr * cc[1].y) / (nl + nr);
-        cz = (nl * cc[0].z + nr * cc[1].z) / (nl + nr);
+        const double nl = node_range[2 * lc + 1] - node_range[2 * lc] + 1;
+        const double nr = node_range[2 * rc + 1] - node_range[2 * rc] + 1;
+        cx = (nl * cl.x + nr * cr.x) / (nl + nr);
+        cy = (nl * cl.y + nr * cr.y) / (nl + nr);
+        cz = (nl * cl.z + nr * cr.z) / (nl + nr);
       }
-      double acc[C];
-      for (int j = 0; j < C; ++j) acc[j] = 0.0;
-      for (int side = 0; side < 2; ++side) {
-        Align<double, P> a;
-        a.init(cc[side].x - cx, cc[side].y - cy, cc[side].z - cz);
-        rotate_all<double, P, Rot::ToZMultipole>(vc[side], a);
-        zshift_m2m<double, P>(vc[side], a.r);
-        rotate_all<double, P, Rot::FromZMultipole>(vc[side], a);
-        for (int j = 0; j < C; ++j) acc[j] += vc[side][j];
+      const int ch = side == 0 ? lc : rc;
+      const V4<double> cc = side == 0 ? cl : cr;
+      double v[C];
+      const int sh = shadow[ch];
+      if (sh >= 0) {
+        load_row<double, C>(mult64 + static_cast<int64_t>(sh) * cs64, v);
+      } else {
+        float tmp[C];
+        load_row<float, C>(mult + static_cast<int64_t>(ch) * cs, tmp);
+#pragma unroll
+        for (int j = 0; j < C; ++j) v[j] = tmp[j];
       }
-      const int sh = shadow[node];
-      store_row<double, C>(mult64 + static_cast<int64_t>(sh) * cs64, acc);
-      store4(centers64, sh, V4<double>{cx, cy, cz, tot});
-      float accf[C];
-      for (int j = 0; j < C; ++j) accf[j] = static_cast<float>(acc[j]);
-      store_row<float, C>(mult + static_cast<int64_t>(node) * cs, accf);
-      store4(centers4, node, V4<float>{static_cast<float>(cx), static_cast<float>(cy), static_cast<float>(cz),
-                                       static_cast<float>(tot)});
+      Align<double, P> a;
+      a.init(cc.x - cx, cc.y - cy, cc.z - cz);  // source - destination
+      rotate_all<double, P, Rot::ToZMultipole>(v, a);
+      zshift_m2m<double, P>(v, a.r);
+      rotate_all<double, P, Rot::FromZMultipole>(v, a);
+#pragma unroll
+      for (int j = 0; j < C; ++j) v[j] = v[j] + __shfl_xor(v[j], 1, kWave);  // left + right on both lanes
+      if (side == 0) {
+        const int sn = shadow[node];
+        store_row<double, C>(mult64 + static_cast<int64_t>(sn) * cs64, v);
+        store4(centers64, sn, V4<double>{cx, cy, cz, tot});
+        float vf[C];
+#pragma unroll
+        for (int j = 0; j < C; ++j) vf[j] = static_cast<float>(v[j]);
+        store_row<float, C>(mult + static_cast<int64_t>(node) * cs, vf);
+        store4(centers4, node, V4<float>{static_cast<float>(cx), static_cast<float>(cy), static_cast<float>(cz),
+                                         static_cast<float>(tot)});
+      }
     }
     __syncthreads();
   }
@@ -547,8 +563,8 @@ int32_t jfm_m2m_top(void* stream, int32_t p, const int32_t* nodes, const int32_t
   if (nlev < 0 || !lev_off_host) return fail(JFM_ERR_ARG, "jfm_m2m_top: bad level table");
   if (nlev == 0) return JFM_OK;
   for (int l = 0; l < nlev; ++l)
-    if (lev_off_host[l + 1] - lev_off_host[l] > kWave || lev_off_host[l + 1] < lev_off_host[l])
-      return fail(JFM_ERR_ARG, "jfm_m2m_top: a top level holds more than 64 nodes");
+    if (lev_off_host[l + 1] - lev_off_host[l] > kWave / 2 || lev_off_host[l + 1] < lev_off_host[l])
+      return fail(JFM_ERR_ARG, "jfm_m2m_top: a top level holds more than 32 nodes (two lanes per node)");
   if (!nodes || !lev_off || !left || !right || !node_range || !shadow || !multipoles || !centers4 || !mult64 ||
       !centers64)
     return fail(JFM_ERR_ARG, "jfm_m2m_top: NULL buffer");
