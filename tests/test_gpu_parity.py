@@ -539,3 +539,19 @@ def test_reuse_prepared_state_ignores_interleaved_prepare(require_gpu):
     fmm.prepare_state(y, m2, leaf_size=16, max_order=4)
     b = fmm.compute_accelerations(x, m, leaf_size=16, max_order=4, reuse_prepared_state=True)
     assert b.shape == a.shape and np.array_equal(a, b)
+
+
+def test_instances_share_the_p2p_stream(require_gpu):
+    """Every instance forks the overlapped step's P2P onto one stream per
+    device, whatever torch's stream pool has handed out in between: a fresh
+    pool stream per instance can share the current stream's hardware queue
+    (then the P2P runs after the tree passes instead of beside them)."""
+    from jaccpot_amd import FastMultipoleMethod
+
+    print("[gpu] shared P2P stream", flush=True)
+    dev = torch.device("cuda", torch.cuda.current_device())
+    first = FastMultipoleMethod(precision="fp32")._side_stream(dev)
+    pool = [torch.cuda.Stream(device=dev) for _ in range(8)]  # advance the pool
+    again = FastMultipoleMethod(precision="fp32")._side_stream(dev)
+    assert again is first and all(s is not first for s in pool)
+    assert first != torch.cuda.current_stream(dev)
