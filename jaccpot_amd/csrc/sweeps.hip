@@ -111,13 +111,38 @@ __global__ __launch_bounds__(kWave) void p2m_kernel(const T* __restrict__ pos4, 
   if (lane == 0) store4(centers4, node, V4<T>{cx, cy, cz, ms});
 }
 
+// The value of another lane of the same row of 16, picked by a DPP control
+// (every lane of the row active): a VALU move instead of an LDS permute.
+// 0xB1 = quad_perm [1,0,3,2] (lane ^ 1), 0x4E = quad_perm [2,3,0,1]
+// (lane ^ 2), 0x141 = row_half_mirror (lane 7 - i of its 8), 0x140 =
+// row_mirror (lane 15 - i of its 16).
+template <int CTRL>
+__device__ __forceinline__ float dpp_get(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_get(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
 // Narrow leaves (leaf_size <= G <= 32): G lanes per leaf, 64 / G leaves per
 // wave.  The same per-leaf quantities as p2m_kernel, summed over the leaf's
 // lanes by an xor tree inside each aligned group of G lanes (fixed order).
+// After the step over 2^k lanes every lane of an aligned 2^k group holds the
+// same sum (a + b and b + a round alike), so the partner of the next step may
+// be any lane of the other half: the mirrors inside a row of 16 give the
+// xor 4 and xor 8 sums bit for bit.
 template <int G, typename T>
 __device__ __forceinline__ T group_sum(T v) {
-#pragma unroll
-  for (int o = 1; o < G; o <<= 1) v += __shfl_xor(v, o, kWave);
+  static_assert(G == 8 || G == 16 || G == 32, "group of 8, 16 or 32 lanes");
+  v += dpp_get<0xB1>(v);
+  v += dpp_get<0x4E>(v);
+  v += dpp_get<0x141>(v);
+  if constexpr (G >= 16) v += dpp_get<0x140>(v);
+  if constexpr (G >= 32) v += __shfl_xor(v, 16, kWave);
   return v;
 }
 
@@ -246,7 +271,7 @@ __global__ __launch_bounds__(kWave) void m2m_kernel(const int32_t* __restrict__ 
   zshift_m2m<T, P>(v, a.r);
   rotate_all<T, P, Rot::FromZMultipole>(v, a);
 #pragma unroll
-  for (int j = 0; j < C; ++j) v[j] = v[j] + __shfl_xor(v[j], 1, kWave);  // left + right on both lanes
+  for (int j = 0; j < C; ++j) v[j] = v[j] + dpp_get<0xB1>(v[j]);  // left + right on both lanes
   if (side == 0 && live) {
     store_row<T, C>(mult + static_cast<int64_t>(node) * cs, v);
     store4(centers4, node, V4<T>{cx, cy, cz, tot});
@@ -380,7 +405,7 @@ __global__ __launch_bounds__(kWave) void m2m_top_kernel(const int32_t* __restric
       zshift_m2m<double, P>(v, a.r);
       rotate_all<double, P, Rot::FromZMultipole>(v, a);
 #pragma unroll
-      for (int j = 0; j < C; ++j) v[j] = v[j] + __shfl_xor(v[j], 1, kWave);  // left + right on both lanes
+      for (int j = 0; j < C; ++j) v[j] = v[j] + dpp_get<0xB1>(v[j]);  // left + right on both lanes
       if (side == 0) {
         const int sn = shadow[node];
         store_row<double, C>(mult64 + static_cast<int64_t>(sn) * cs64, v);
