@@ -114,21 +114,36 @@ def adjoint_gears(state) -> AdjointGears:
     return out
 
 
-def _recompute(fmm, state, positions: torch.Tensor, masses: torch.Tensor):
-    """Forward expansions at the live inputs with the forward's own passes
-    (learned gears, float64 top levels); returns the locals after M2L only
-    (the full locals are left in ``state.locals``)."""
-    topo = state.topology
-    s = nat.current_stream()
-    code = nat.dtype_code(state.dtype)
-    nat.call("jfm_gather_sorted", s, nat.dtype_code(positions.dtype), code, positions.data_ptr(),
-             masses.data_ptr(), state.perm.data_ptr(), topo.num_particles, state.pos4.data_ptr())
+def _recompute(fmm, state, s, code):
+    """Forward expansions at the live inputs (already gathered into
+    ``state.pos4``) with the forward's own passes (learned gears, float64 top
+    levels); returns the locals after M2L only (the full locals are left in
+    ``state.locals``)."""
     fmm._upward(state, s, code)
     state.locals.zero_()
     fmm._m2l(state, s, code)
     lown = state.locals.clone()
     fmm._l2l(state, s, code)
     return lown
+
+
+def _p2p_adjoint(state, code, s, cot4, grad4):
+    """Near-field part of the particle gradient into grad4 (written, not added)."""
+    topo = state.topology
+    n = topo.num_particles
+    items = state.extra.get("p2p_items")
+    if items is None:
+        nat.call("jfm_p2p_adjoint", s, code, state.pos4.data_ptr(), cot4.data_ptr(), n, topo.num_leaves,
+                 topo.leaf_size, state.near_off.data_ptr(), state.near_src.data_ptr(), state.softening ** 2, state.G,
+                 grad4.data_ptr())
+    else:  # long near lists cut into work items, as in the forward P2P
+        nat.call("jfm_p2p_adjoint_items", s, code, state.pos4.data_ptr(), cot4.data_ptr(), n, topo.num_leaves,
+                 topo.leaf_size, items.num_items, items.leaf.data_ptr(), items.s0.data_ptr(), items.s1.data_ptr(),
+                 items.part.data_ptr(), items.num_part, items.partials.data_ptr(), state.near_off.data_ptr(),
+                 state.near_src.data_ptr(), state.softening ** 2, state.G, grad4.data_ptr())
+        nat.call("jfm_p2p_adjoint_reduce", s, code, cot4.data_ptr(), n, topo.leaf_size, items.num_split,
+                 items.split_leaf.data_ptr(), items.split_p0.data_ptr(), items.split_p1.data_ptr(),
+                 items.partials.data_ptr(), state.softening ** 2, state.G, grad4.data_ptr())
 
 
 def fmm_vjp(fmm, state, positions: torch.Tensor, masses: torch.Tensor, acc_bar: torch.Tensor,
@@ -143,30 +158,30 @@ def fmm_vjp(fmm, state, positions: torch.Tensor, masses: torch.Tensor, acc_bar: 
     x = positions.to(dev, dt).contiguous()
     m = masses.to(dev, dt).contiguous()
     ag = adjoint_gears(state) if state.extra.get("gears") is not None else None
-    lown = _recompute(fmm, state, x, m)
     n = topo.num_particles
+    # tree-order particles and cotangents: all that the adjoint P2P reads
+    nat.call("jfm_gather_sorted", s, code, code, x.data_ptr(), m.data_ptr(), state.perm.data_ptr(), n,
+             state.pos4.data_ptr())
     cot4 = torch.empty((n, 4), dtype=dt, device=dev)
     ab = acc_bar.to(dev, dt).contiguous()
     pb = pot_bar.to(dev, dt).contiguous()
     nat.call("jfm_gather_sorted", s, code, code, ab.data_ptr(), pb.data_ptr(), state.perm.data_ptr(), n,
              cot4.data_ptr())
     grad4 = torch.empty((n, 4), dtype=dt, device=dev)
-    items = state.extra.get("p2p_items")
-    if items is None:
-        nat.call("jfm_p2p_adjoint", s, code, state.pos4.data_ptr(), cot4.data_ptr(), n, topo.num_leaves,
-                 topo.leaf_size, state.near_off.data_ptr(), state.near_src.data_ptr(), state.softening ** 2, state.G,
-                 grad4.data_ptr())
-    else:  # long near lists cut into work items, as in the forward P2P
-        nat.call("jfm_p2p_adjoint_items", s, code, state.pos4.data_ptr(), cot4.data_ptr(), n, topo.num_leaves,
-                 topo.leaf_size, items.num_items, items.leaf.data_ptr(), items.s0.data_ptr(), items.s1.data_ptr(),
-                 items.part.data_ptr(), items.num_part, items.partials.data_ptr(), state.near_off.data_ptr(),
-                 state.near_src.data_ptr(), state.softening ** 2, state.G, grad4.data_ptr())
-        nat.call("jfm_p2p_adjoint_reduce", s, code, cot4.data_ptr(), n, topo.leaf_size, items.num_split,
-                 items.split_leaf.data_ptr(), items.split_p0.data_ptr(), items.split_p1.data_ptr(),
-                 items.partials.data_ptr(), state.softening ** 2, state.G, grad4.data_ptr())
+    # the adjoint P2P (most of the work) runs on the overlapped step's P2P
+    # stream while this stream recomputes the expansions and runs the tree
+    # adjoints; the L2P part of the particle gradient goes to its own rows
+    # and is added to the P2P rows after the join
+    main = torch.cuda.current_stream(dev)
+    side = fmm._side_stream(dev)
+    side.wait_stream(main)
+    with torch.cuda.stream(side):
+        _p2p_adjoint(state, code, nat.current_stream(), cot4, grad4)
+    lown = _recompute(fmm, state, s, code)
+    grad_l4 = torch.zeros((n, 4), dtype=dt, device=dev)
     lbar = torch.zeros_like(state.locals)
     nat.call("jfm_leaf_adjoint_a", s, code, state.order, state.pos4.data_ptr(), cot4.data_ptr(), n, topo.leaf_size,
-             topo.num_internal, state.locals.data_ptr(), state.centers4.data_ptr(), state.G, grad4.data_ptr(),
+             topo.num_internal, state.locals.data_ptr(), state.centers4.data_ptr(), state.G, grad_l4.data_ptr(),
              lbar.data_ptr())
     # cotangents of the locals up the tree (l2l^T = m2m on parity-flipped rows)
     off = topo.m2m_off
@@ -206,6 +221,8 @@ def fmm_vjp(fmm, state, positions: torch.Tensor, masses: torch.Tensor, acc_bar: 
     chain = torch.empty((topo.num_leaves, 8), dtype=dt, device=dev)
     nat.call("jfm_com_chain", s, code, topo.num_leaves, topo.num_internal, state.parent.data_ptr(),
              state.node_range.data_ptr(), state.centers4.data_ptr(), cbar4.data_ptr(), chain.data_ptr())
+    main.wait_stream(side)
+    grad4.add_(grad_l4)
     gpos = torch.empty((n, 3), dtype=dt, device=dev)
     gmass = torch.empty((n,), dtype=dt, device=dev)
     nat.call("jfm_leaf_adjoint_b", s, code, state.order, state.pos4.data_ptr(), n, topo.leaf_size, topo.num_internal,
@@ -219,11 +236,12 @@ class _FMMForce(torch.autograd.Function):
     def forward(ctx, positions, masses, fmm, state):
         x = positions.detach().to(state.device, state.dtype).contiguous()
         m = masses.detach().to(state.device, state.dtype).contiguous()
-        fmm.refresh_prepared_state(state, x, m)
         n = state.num_particles
         acc = torch.empty((n, 3), dtype=state.dtype, device=state.device)
         pot = torch.empty((n,), dtype=state.dtype, device=state.device)
-        fmm._evaluate_device(state, acc, pot)
+        # refresh + evaluation with the P2P beside the tree passes (the same
+        # bits as refresh_prepared_state followed by the one-stream evaluation)
+        fmm.step_device(state, x, m, acc, pot)
         ctx.fmm = fmm
         ctx.state = state
         ctx.save_for_backward(x, m)
