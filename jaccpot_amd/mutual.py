@@ -371,16 +371,18 @@ def _far_m2l(st: MutualFMMState, lw, src: torch.Tensor, dst: torch.Tensor, code:
                  fl.span_tgt.data_ptr(), fl.span_w0.data_ptr(), fl.span_w1.data_ptr())
 
 
-def _expansions(st: MutualFMMState, x: torch.Tensor, m: torch.Tensor, lw) -> torch.Tensor:
-    """Gather, P2M, M2M, (weighted) M2L, L2L at the live inputs; returns the
-    locals after M2L alone (the full locals stay in the prepared state)."""
+def _expansions(st: MutualFMMState, x: torch.Tensor, m: torch.Tensor, lw, gathered: bool = False) -> torch.Tensor:
+    """Gather (unless ``gathered``), P2M, M2M, (weighted) M2L, L2L at the live
+    inputs; returns the locals after M2L alone (the full locals stay in the
+    prepared state)."""
     prep = st.prepared
     topo = prep.topology
     s = nat.current_stream()
     code = nat.dtype_code(prep.dtype)
     n = topo.num_particles
-    nat.call("jfm_gather_sorted", s, code, code, x.data_ptr(), m.data_ptr(), prep.perm.data_ptr(), n,
-             prep.pos4.data_ptr())
+    if not gathered:
+        nat.call("jfm_gather_sorted", s, code, code, x.data_ptr(), m.data_ptr(), prep.perm.data_ptr(), n,
+                 prep.pos4.data_ptr())
     nat.call("jfm_p2m", s, code, prep.order, prep.pos4.data_ptr(), n, topo.num_leaves, topo.leaf_size,
              topo.num_internal, prep.multipoles.data_ptr(), prep.centers4.data_ptr())
     _tree_level(prep, prep.m2m_nodes, topo.m2m_off, "m2m", prep.multipoles, code, s)
@@ -398,11 +400,35 @@ def _evaluate(st: MutualFMMState, x: torch.Tensor, m: torch.Tensor, lw):
     s = nat.current_stream()
     code = nat.dtype_code(prep.dtype)
     n, L, ls, I = topo.num_particles, topo.num_leaves, topo.leaf_size, topo.num_internal
-    _expansions(st, x, m, lw)
+    nat.call("jfm_gather_sorted", s, code, code, x.data_ptr(), m.data_ptr(), prep.perm.data_ptr(), n,
+             prep.pos4.data_ptr())
+    near = torch.empty((n, 3), dtype=prep.dtype, device=prep.device)
+    # the near field (reads only the gathered particles) runs on the P2P
+    # stream of the device beside the tree passes and L2P on this one
+    from .solver import FastMultipoleMethod
+
+    main = torch.cuda.current_stream(prep.device)
+    side = FastMultipoleMethod._side_stream(prep.device)
+    side.wait_stream(main)
+    with torch.cuda.stream(side):
+        _near_field(st, lw, near, code, nat.current_stream())
+    _expansions(st, x, m, lw, gathered=True)
     far_acc = st.buffers["far_acc"]
     nat.call("jfm_l2p", s, code, prep.order, prep.pos4.data_ptr(), n, ls, I, prep.locals.data_ptr(),
              prep.centers4.data_ptr(), st.G, prep.perm.data_ptr(), far_acc.data_ptr(), None, 0)
-    near = torch.empty((n, 3), dtype=prep.dtype, device=prep.device)
+    main.wait_stream(side)
+    acc = torch.empty((n, 3), dtype=prep.dtype, device=prep.device)
+    far = torch.empty((n, 3), dtype=prep.dtype, device=prep.device)
+    nat.call("jfm_mutual_combine", s, code, n, m.data_ptr(), far_acc.data_ptr(), near.data_ptr(), acc.data_ptr(),
+             far.data_ptr())
+    return acc, near, far
+
+
+def _near_field(st: MutualFMMState, lw, near: torch.Tensor, code: int, s) -> None:
+    """Rung-weighted mutual near field of the gathered particles into ``near`` (input order)."""
+    prep = st.prepared
+    topo = prep.topology
+    n, L, ls = topo.num_particles, topo.num_leaves, topo.leaf_size
     weighted = lw is not None
     rs = st.buffers["rung_sorted"].data_ptr() if weighted else None
     lwp, nlev = (lw.data_ptr(), int(lw.numel())) if weighted else (None, 0)
@@ -417,11 +443,6 @@ def _evaluate(st: MutualFMMState, x: torch.Tensor, m: torch.Tensor, lw):
                  it.num_split, it.split_leaf.data_ptr(), it.split_p0.data_ptr(), it.split_p1.data_ptr(),
                  prep.near_off.data_ptr(), prep.near_src.data_ptr(), rs, lwp, nlev, st.softening ** 2, st.G,
                  prep.perm.data_ptr(), near.data_ptr())
-    acc = torch.empty((n, 3), dtype=prep.dtype, device=prep.device)
-    far = torch.empty((n, 3), dtype=prep.dtype, device=prep.device)
-    nat.call("jfm_mutual_combine", s, code, n, m.data_ptr(), far_acc.data_ptr(), near.data_ptr(), acc.data_ptr(),
-             far.data_ptr())
-    return acc, near, far
 
 
 def _vjp(st: MutualFMMState, x: torch.Tensor, m: torch.Tensor, lw, acc_bar: torch.Tensor):
