@@ -445,6 +445,34 @@ def _near_field(st: MutualFMMState, lw, near: torch.Tensor, code: int, s) -> Non
                  prep.perm.data_ptr(), near.data_ptr())
 
 
+def _near_adjoint(st: MutualFMMState, lw, cot4, grad4, part, code: int, s) -> None:
+    """Near-field part of the particle gradient into grad4 (written, not added);
+    ``part``: partials of its own for the work items (None without items)."""
+    prep = st.prepared
+    topo = prep.topology
+    n, L, ls = topo.num_particles, topo.num_leaves, topo.leaf_size
+    eps2 = st.softening ** 2
+    it = st.near_items
+    if it is None and lw is not None:
+        nat.call("jfm_p2p_adjoint_weighted", s, code, prep.pos4.data_ptr(), cot4.data_ptr(), n, L, ls,
+                 prep.near_off.data_ptr(), prep.near_src.data_ptr(), st.buffers["rung_sorted"].data_ptr(),
+                 lw.data_ptr(), int(lw.numel()), eps2, st.G, grad4.data_ptr())
+    elif it is None:
+        nat.call("jfm_p2p_adjoint", s, code, prep.pos4.data_ptr(), cot4.data_ptr(), n, L, ls,
+                 prep.near_off.data_ptr(), prep.near_src.data_ptr(), eps2, st.G, grad4.data_ptr())
+    else:  # long near lists cut into the forward's work items
+        head = (s, code, prep.pos4.data_ptr(), cot4.data_ptr(), n, L, ls, it.num_items, it.leaf.data_ptr(),
+                it.s0.data_ptr(), it.s1.data_ptr(), it.part.data_ptr(), it.num_part, part.data_ptr(),
+                prep.near_off.data_ptr(), prep.near_src.data_ptr())
+        if lw is not None:
+            nat.call("jfm_p2p_adjoint_weighted_items", *head, st.buffers["rung_sorted"].data_ptr(), lw.data_ptr(),
+                     int(lw.numel()), eps2, st.G, grad4.data_ptr())
+        else:
+            nat.call("jfm_p2p_adjoint_items", *head, eps2, st.G, grad4.data_ptr())
+        nat.call("jfm_p2p_adjoint_reduce", s, code, cot4.data_ptr(), n, ls, it.num_split, it.split_leaf.data_ptr(),
+                 it.split_p0.data_ptr(), it.split_p1.data_ptr(), part.data_ptr(), eps2, st.G, grad4.data_ptr())
+
+
 def _vjp(st: MutualFMMState, x: torch.Tensor, m: torch.Tensor, lw, acc_bar: torch.Tensor):
     """(positions_bar, masses_bar) in input order for the cotangent of the mutual
     accelerations.  For m_i > 0 the acceleration is the level-weighted FMM
@@ -457,37 +485,30 @@ def _vjp(st: MutualFMMState, x: torch.Tensor, m: torch.Tensor, lw, acc_bar: torc
     code = nat.dtype_code(prep.dtype)
     n, L, ls, I = topo.num_particles, topo.num_leaves, topo.leaf_size, topo.num_internal
     dt, dev = prep.dtype, prep.device
-    lown = _expansions(st, x, m, lw)
+    nat.call("jfm_gather_sorted", s, code, code, x.data_ptr(), m.data_ptr(), prep.perm.data_ptr(), n,
+             prep.pos4.data_ptr())
     ab = torch.where((m > 0)[:, None], acc_bar.to(dev, dt), torch.zeros((), dtype=dt, device=dev)).contiguous()
     pb = torch.zeros(n, dtype=dt, device=dev)
     cot4 = torch.empty((n, 4), dtype=dt, device=dev)
     nat.call("jfm_gather_sorted", s, code, code, ab.data_ptr(), pb.data_ptr(), prep.perm.data_ptr(), n,
              cot4.data_ptr())
     grad4 = torch.empty((n, 4), dtype=dt, device=dev)
-    eps2 = st.softening ** 2
-    it = st.near_items
-    if it is None and lw is not None:
-        nat.call("jfm_p2p_adjoint_weighted", s, code, prep.pos4.data_ptr(), cot4.data_ptr(), n, L, ls,
-                 prep.near_off.data_ptr(), prep.near_src.data_ptr(), st.buffers["rung_sorted"].data_ptr(),
-                 lw.data_ptr(), int(lw.numel()), eps2, st.G, grad4.data_ptr())
-    elif it is None:
-        nat.call("jfm_p2p_adjoint", s, code, prep.pos4.data_ptr(), cot4.data_ptr(), n, L, ls,
-                 prep.near_off.data_ptr(), prep.near_src.data_ptr(), eps2, st.G, grad4.data_ptr())
-    else:  # long near lists cut into the forward's work items; partials of its own
-        part = torch.empty_like(it.partials)
-        head = (s, code, prep.pos4.data_ptr(), cot4.data_ptr(), n, L, ls, it.num_items, it.leaf.data_ptr(),
-                it.s0.data_ptr(), it.s1.data_ptr(), it.part.data_ptr(), it.num_part, part.data_ptr(),
-                prep.near_off.data_ptr(), prep.near_src.data_ptr())
-        if lw is not None:
-            nat.call("jfm_p2p_adjoint_weighted_items", *head, st.buffers["rung_sorted"].data_ptr(), lw.data_ptr(),
-                     int(lw.numel()), eps2, st.G, grad4.data_ptr())
-        else:
-            nat.call("jfm_p2p_adjoint_items", *head, eps2, st.G, grad4.data_ptr())
-        nat.call("jfm_p2p_adjoint_reduce", s, code, cot4.data_ptr(), n, ls, it.num_split, it.split_leaf.data_ptr(),
-                 it.split_p0.data_ptr(), it.split_p1.data_ptr(), part.data_ptr(), eps2, st.G, grad4.data_ptr())
+    part = torch.empty_like(st.near_items.partials) if st.near_items is not None else None
+    # the adjoint near field runs on the P2P stream of the device while this
+    # stream recomputes the expansions and runs the tree adjoints; the L2P
+    # part of the particle gradient goes to its own rows, added after the join
+    from .solver import FastMultipoleMethod
+
+    main = torch.cuda.current_stream(dev)
+    side = FastMultipoleMethod._side_stream(dev)
+    side.wait_stream(main)
+    with torch.cuda.stream(side):
+        _near_adjoint(st, lw, cot4, grad4, part, code, nat.current_stream())
+    lown = _expansions(st, x, m, lw, gathered=True)
+    grad_l4 = torch.zeros((n, 4), dtype=dt, device=dev)
     lbar = torch.zeros_like(prep.locals)
     nat.call("jfm_leaf_adjoint_a", s, code, prep.order, prep.pos4.data_ptr(), cot4.data_ptr(), n, ls, I,
-             prep.locals.data_ptr(), prep.centers4.data_ptr(), st.G, grad4.data_ptr(), lbar.data_ptr())
+             prep.locals.data_ptr(), prep.centers4.data_ptr(), st.G, grad_l4.data_ptr(), lbar.data_ptr())
     _tree_level(prep, prep.m2m_nodes, topo.m2m_off, "m2m", lbar, code, s)  # l2l^T
     mbar = torch.zeros_like(prep.multipoles)
     cbar4 = torch.zeros((topo.num_nodes, 4), dtype=dt, device=dev)
@@ -507,6 +528,8 @@ def _vjp(st: MutualFMMState, x: torch.Tensor, m: torch.Tensor, lw, acc_bar: torc
     chain = torch.empty((L, 8), dtype=dt, device=dev)
     nat.call("jfm_com_chain", s, code, L, I, prep.parent.data_ptr(), prep.node_range.data_ptr(),
              prep.centers4.data_ptr(), cbar4.data_ptr(), chain.data_ptr())
+    main.wait_stream(side)
+    grad4.add_(grad_l4)
     gpos = torch.empty((n, 3), dtype=dt, device=dev)
     gmass = torch.empty((n,), dtype=dt, device=dev)
     nat.call("jfm_leaf_adjoint_b", s, code, prep.order, prep.pos4.data_ptr(), n, ls, I, mbar.data_ptr(),
